@@ -1,0 +1,316 @@
+"""chaos-ultra for MI355X — Python view of the C ABI (include/chaosultra.h).
+
+Import with ``importlib.import_module("chaos-ultra_amd")`` (the directory name has a
+hyphen).  Everything here forwards to lib/libchaosultra_hip.so; device work runs in
+the gfx950 code objects under kernels/.  The classes mirror the reference's plugin
+surface: FractalRendererProvider / FractalRenderer (rendering/FractalRenderer.java,
+rendering/FractalRendererProvider.java) and the RenderingModel it renders from.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import (  # noqa: F401  (re-exported)
+    KERNELS_DIR, LIB_PATH, ChaosError, CudaInitializationError, DeviceError, FractalRendererError,
+    IllegalArgumentError, IllegalStateError, RenderingModel, TileDesc, Tune,
+    FLAG_ADAPTIVE_SS, FLAG_FOVEATION, FLAG_SAMPLE_REUSE, FLAG_VISUALISE_SAMPLE_COUNT,
+    FLAG_ZOOMING, FLAG_ZOOMING_IN, FRACTAL_JULIA, FRACTAL_MANDELBROT, PRECISION_DOUBLE,
+    PRECISION_SINGLE, PRECISION_TOO_BIG, SAMPLE_CENTRE, SAMPLE_CORNER, SAMPLE_JITTER,
+    STATE_NOT_INITIALIZED, STATE_READY_TO_RENDER,
+)
+
+MAX_SUPER_SAMPLING = 64  # FractalRenderer.MAX_SUPER_SAMPLING
+
+PIXEL_INFO_DTYPE = np.dtype([("value", "<f4"), ("weight", "<f4"), ("is_reused", "u1"),
+                             ("pad", "u1", (3,)), ("weight_of_new_samples", "<f4")])
+assert PIXEL_INFO_DTYPE.itemsize == 16
+
+
+def init(device: int = 0) -> None:
+    abi.call("cu_init", device)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    abi.call("cu_device_count", C.byref(n))
+    return n.value
+
+
+def synchronize() -> None:
+    abi.call("cu_ctx_synchronize")
+
+
+# ---------------------------------------------------------------------------
+# device memory
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """A hipMalloc'd buffer freed on close()/garbage collection."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        abi.call("cu_mem_alloc", C.byref(self.ptr), C.c_size_t(self.nbytes))
+
+    @classmethod
+    def from_array(cls, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        b.upload(arr)
+        return b
+
+    @property
+    def address(self) -> int:
+        return self.ptr.value or 0
+
+    def upload(self, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array larger than buffer")
+        abi.call("cu_memcpy_htod", self.ptr, arr.ctypes.data_as(C.c_void_p), C.c_size_t(arr.nbytes))
+
+    def download(self, dtype, count: int | None = None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        n = self.nbytes // dtype.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dtype)
+        abi.call("cu_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(out.nbytes))
+        return out
+
+    def fill_u32(self, value: int = 0) -> None:
+        abi.call("cu_memset_u32", self.ptr, C.c_uint32(value), C.c_size_t(self.nbytes // 4))
+
+    def close(self) -> None:
+        if self.ptr and self.ptr.value:
+            abi.call("cu_mem_free", self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        self.handle = C.c_void_p()
+        abi.call("cu_stream_create", C.byref(self.handle))
+
+    def synchronize(self) -> None:
+        abi.call("cu_stream_synchronize", self.handle)
+
+    def done(self) -> bool:
+        d = C.c_int(0)
+        abi.call("cu_stream_query", self.handle, C.byref(d))
+        return bool(d.value)
+
+    def close(self) -> None:
+        if self.handle and self.handle.value:
+            abi.call("cu_stream_destroy", self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# renderer layer
+# ---------------------------------------------------------------------------
+def plane_segment_from_center(cx: float, cy: float, zoom: float, width: int, height: int):
+    """Model.setPlaneSegmentFromCenter: zoom is the segment height."""
+    out = (C.c_double * 4)()
+    abi.call("cu_plane_segment_from_center", C.c_double(cx), C.c_double(cy), C.c_double(zoom),
+             width, height, out)
+    return tuple(out)
+
+
+def precision_for_segment(lbx, lby, rtx, rty, width, height) -> int:
+    p = C.c_int(0)
+    abi.call("cu_precision_for_segment", C.c_double(lbx), C.c_double(lby), C.c_double(rtx),
+             C.c_double(rty), width, height, C.byref(p))
+    return p.value
+
+
+def launch_geometry(width: int, height: int):
+    gx, gy = C.c_uint(0), C.c_uint(0)
+    abi.call("cu_launch_geometry", width, height, C.byref(gx), C.byref(gy))
+    return gx.value, gy.value
+
+
+def default_palette() -> np.ndarray:
+    out = np.zeros(1536, dtype=np.uint32)
+    abi.call("cu_default_palette", out.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_size_t(1536))
+    return out
+
+
+def palette_from_argb(argb) -> np.ndarray:
+    argb = np.ascontiguousarray(argb, dtype=np.uint32)
+    out = np.zeros_like(argb)
+    abi.call("cu_palette_from_argb", argb.ctypes.data_as(C.POINTER(C.c_uint32)),
+             out.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_size_t(argb.size))
+    return out
+
+
+class FractalRenderer:
+    """FractalRenderer (rendering/FractalRenderer.java) backed by a native renderer handle."""
+
+    def __init__(self, handle: int):
+        self.handle = handle
+
+    def initialize_rendering(self, width: int, height: int, palette: np.ndarray) -> None:
+        self._palette = np.ascontiguousarray(palette, dtype=np.uint32)
+        abi.call("cu_renderer_initialize_rendering", self.handle, width, height,
+                 self._palette.ctypes.data_as(C.POINTER(C.c_uint32)), int(self._palette.size))
+
+    def free_rendering_resources(self) -> None:
+        abi.call("cu_renderer_free_rendering_resources", self.handle)
+
+    def get_state(self) -> int:
+        s = C.c_int(0)
+        abi.call("cu_renderer_get_state", self.handle, C.byref(s))
+        return s.value
+
+    def get_width(self) -> int:
+        v = C.c_int(0)
+        abi.call("cu_renderer_get_width", self.handle, C.byref(v))
+        return v.value
+
+    def get_height(self) -> int:
+        v = C.c_int(0)
+        abi.call("cu_renderer_get_height", self.handle, C.byref(v))
+        return v.value
+
+    def launch_debug_kernel(self) -> None:
+        abi.call("cu_renderer_launch_debug_kernel", self.handle)
+
+    def render_fast(self, model: RenderingModel) -> None:
+        abi.call("cu_renderer_render_fast", self.handle, C.byref(model))
+
+    def render_quality(self, model: RenderingModel) -> None:
+        abi.call("cu_renderer_render_quality", self.handle, C.byref(model))
+
+    def close(self) -> None:
+        abi.call("cu_renderer_close", self.handle)
+
+    def set_fractal_custom_params(self, text: str) -> None:
+        abi.call("cu_renderer_set_fractal_custom_params", self.handle, text.encode())
+
+    def get_fractal_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        abi.call("cu_renderer_get_fractal_name", self.handle, buf, C.c_size_t(256))
+        return buf.value.decode()
+
+    def supply_default_values(self, model: RenderingModel, canvas_width: int, canvas_height: int) -> None:
+        abi.call("cu_renderer_supply_default_values", self.handle, C.byref(model), canvas_width,
+                 canvas_height)
+
+    def read_output(self) -> np.ndarray:
+        w, h = self.get_width(), self.get_height()
+        out = np.zeros(w * h, dtype=np.uint32)
+        abi.call("cu_renderer_read_output", self.handle, out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                 C.c_size_t(out.size))
+        return out.reshape(h, w)
+
+    def read_pixel_info(self, primary: bool = True) -> np.ndarray:
+        w, h = self.get_width(), self.get_height()
+        out = np.zeros(w * h, dtype=PIXEL_INFO_DTYPE)
+        abi.call("cu_renderer_read_pixel_info", self.handle, 1 if primary else 0,
+                 out.ctypes.data_as(C.c_void_p), C.c_size_t(out.nbytes))
+        return out.reshape(h, w)
+
+
+class FractalRendererProvider:
+    """CudaFractalRendererProvider counterpart: modules are <kernels_dir>/<name>.hsaco."""
+
+    def __init__(self, kernels_dir: str = KERNELS_DIR):
+        h = C.c_int32(0)
+        abi.call("cu_provider_create", kernels_dir.encode(), C.byref(h))
+        self.handle = h.value
+
+    def get_available_fractals(self) -> set:
+        buf = C.create_string_buffer(1024)
+        abi.call("cu_provider_available_fractals", self.handle, buf, C.c_size_t(1024))
+        return set(buf.value.decode().split(",")) if buf.value else set()
+
+    def get_default_renderer(self) -> FractalRenderer:
+        h = C.c_int32(0)
+        abi.call("cu_provider_get_default_renderer", self.handle, C.byref(h))
+        return FractalRenderer(h.value)
+
+    def get_renderer(self, name: str, force_reload: bool = False) -> FractalRenderer:
+        h = C.c_int32(0)
+        abi.call("cu_provider_get_renderer", self.handle, name.encode(), 1 if force_reload else 0,
+                 C.byref(h))
+        return FractalRenderer(h.value)
+
+    def close(self) -> None:
+        if self.handle:
+            abi.call("cu_provider_destroy", self.handle)
+            self.handle = 0
+
+
+def set_debug_mode(enabled: bool) -> None:
+    abi.call("cu_set_debug_mode", 1 if enabled else 0)
+
+
+# ---------------------------------------------------------------------------
+# escape-family layer
+# ---------------------------------------------------------------------------
+def make_tile(lbx, lby, rtx, rty, width, height, max_iter, bailout_sq=4.0,
+              kind=FRACTAL_MANDELBROT, julia_c=(0.0, 0.0), spp=1, pattern=SAMPLE_CENTRE,
+              seed=0) -> TileDesc:
+    t = TileDesc()
+    abi.call("cu_tile_init", C.byref(t), C.c_double(lbx), C.c_double(lby), C.c_double(rtx),
+             C.c_double(rty), width, height, max_iter, C.c_double(bailout_sq), kind,
+             C.c_double(julia_c[0]), C.c_double(julia_c[1]), spp, pattern, C.c_uint64(seed))
+    return t
+
+
+def tile_fused_safe(tile: TileDesc, precision: int) -> bool:
+    s = C.c_int(0)
+    abi.call("cu_tile_fused_safe", C.byref(tile), precision, C.byref(s))
+    return bool(s.value)
+
+
+class Engine:
+    """The escape-family module (kernels/escape_family.hsaco)."""
+
+    def __init__(self, kernels_dir: str = KERNELS_DIR):
+        self.handle = C.c_void_p()
+        abi.call("cu_engine_create", kernels_dir.encode(), C.byref(self.handle))
+
+    def escape_tile(self, precision, tile: TileDesc, counts: DeviceBuffer, smooth: DeviceBuffer | None,
+                    iters: DeviceBuffer | None, variant: int = -1, stream: Stream | None = None):
+        abi.call("cu_escape_tile", self.handle, precision, variant, C.byref(tile), counts.ptr,
+                 smooth.ptr if smooth else None, iters.ptr if iters else None,
+                 stream.handle if stream else None)
+
+    def escape_tiles_persistent(self, precision, variant, tiles: DeviceBuffer, n_tiles: int,
+                                total: int, counts: DeviceBuffer, smooth: DeviceBuffer | None,
+                                iters: DeviceBuffer | None, queue: DeviceBuffer,
+                                tune: Tune | None = None, stream: Stream | None = None):
+        abi.call("cu_escape_tiles_persistent", self.handle, precision, variant, tiles.ptr, n_tiles,
+                 C.c_uint64(total), counts.ptr, smooth.ptr if smooth else None,
+                 iters.ptr if iters else None, queue.ptr, C.byref(tune) if tune else None,
+                 stream.handle if stream else None)
+
+    def compose_smooth(self, smooth: DeviceBuffer, smooth_pitch, rgba: DeviceBuffer, rgba_pitch,
+                       width, height, palette: DeviceBuffer, palette_length,
+                       stream: Stream | None = None):
+        abi.call("cu_compose_smooth", self.handle, smooth.ptr, smooth_pitch, rgba.ptr, rgba_pitch,
+                 width, height, palette.ptr, palette_length, stream.handle if stream else None)
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float(0)
+        abi.call("cu_engine_last_kernel_ms", self.handle, C.byref(ms))
+        return ms.value
+
+    def close(self) -> None:
+        if self.handle and self.handle.value:
+            abi.call("cu_engine_destroy", self.handle)
+            self.handle = C.c_void_p()

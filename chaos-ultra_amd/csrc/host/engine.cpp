@@ -1,0 +1,257 @@
+// engine.cpp — layer 3 of the C ABI: launches of the escape-family module
+// (escape_family.hsaco) for the planar tile forms, the persistent launch and
+// the smooth compose, with HIP-event timing on the launch stream.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "common.hpp"
+#include "handles.hpp"
+
+struct cu_engine_s {
+    cu_module module = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool timed = false;
+    std::mutex mutex;
+};
+
+namespace cu {
+namespace {
+
+constexpr uint32_t kMaxPersistentTiles = 1024;  // tile table staged in LDS
+
+cu_function function(cu_engine e, const std::string &name) {
+    cu_function f = nullptr;
+    int rc = cu_module_get_function(e->module, name.c_str(), &f);
+    if (rc != CU_OK) fail(rc, cu_last_error());
+    return f;
+}
+
+hipStream_t stream_of(cu_stream s) { return s ? s->stream : nullptr; }
+
+void launch(cu_engine e, cu_function f, dim3 grid, dim3 block, void **params, cu_stream s,
+            bool timed) {
+    hipStream_t st = stream_of(s);
+    if (timed) check(hipEventRecord(e->start, st), "hipEventRecord");
+    check(hipModuleLaunchKernel(f->function, grid.x, grid.y, grid.z, block.x, block.y, block.z, 0,
+                                st, params, nullptr),
+          (std::string("hipModuleLaunchKernel ") + f->name).c_str());
+    if (timed) {
+        check(hipEventRecord(e->stop, st), "hipEventRecord");
+        e->timed = true;
+    }
+}
+
+const char *real_name(int precision) {
+    if (precision == 64) return "Double";
+    if (precision == 32) return "Float";
+    fail(CU_ERR_ILLEGAL_ARGUMENT, "precision must be 32 or 64");
+}
+
+bool ok_magnitude(double v, double threshold) { return v == 0 || std::fabs(v) >= threshold; }
+
+// Sufficient condition for the fused step: the plane origin, pixel sizes and a
+// Julia c are each zero or far from the subnormal range, so every sample point
+// and orbit component is zero or at least ~2^-360 (FP64) / ~2^-35 (FP32) in
+// magnitude and no product x*y of the orbit reaches the subnormal range.
+bool fused_safe(const cu_tile_desc &t, int precision) {
+    if (precision == 64) {
+        const double T = std::ldexp(1.0, -300);
+        bool ok = ok_magnitude(t.left, T) && ok_magnitude(t.top, T) && t.pixel_w >= T &&
+                  t.pixel_h >= T;
+        if (t.kind == CU_FRACTAL_JULIA) ok = ok && ok_magnitude(t.julia_cx, T) && ok_magnitude(t.julia_cy, T);
+        return ok;
+    }
+    const double T = std::ldexp(1.0, -11);
+    bool ok = std::fabs((double)t.left_f) >= T && std::fabs((double)t.top_f) >= T;
+    if (t.kind == CU_FRACTAL_JULIA)
+        ok = ok && ok_magnitude((double)(float)t.julia_cx, T) && ok_magnitude((double)(float)t.julia_cy, T);
+    return ok;
+}
+
+void validate_tile(const cu_tile_desc &t) {
+    if (t.width == 0 || t.height == 0) fail(CU_ERR_ILLEGAL_ARGUMENT, "tile has no pixels");
+    if (t.spp == 0 || t.spp > 4096) fail(CU_ERR_ILLEGAL_ARGUMENT, "spp must be in [1, 4096]");
+    if (t.max_iter == 0) fail(CU_ERR_ILLEGAL_ARGUMENT, "max_iter must be positive");
+    if (t.kind > CU_FRACTAL_JULIA) fail(CU_ERR_ILLEGAL_ARGUMENT, "unknown fractal kind");
+    if (t.pattern > CU_SAMPLE_CORNER) fail(CU_ERR_ILLEGAL_ARGUMENT, "unknown sample pattern");
+    if (t.out_pitch < t.width) fail(CU_ERR_ILLEGAL_ARGUMENT, "out_pitch smaller than width");
+    for (double v : {t.left, t.top, t.pixel_w, t.pixel_h, t.julia_cx, t.julia_cy, t.bailout_sq})
+        if (!std::isfinite(v)) fail(CU_ERR_ILLEGAL_ARGUMENT, "tile parameters must be finite");
+    if (!(t.bailout_sq > 0)) fail(CU_ERR_ILLEGAL_ARGUMENT, "bailout_sq must be positive");
+}
+
+bool pick_fused(int variant, const cu_tile_desc &t, int precision) {
+    if (variant == 0) return false;
+    if (variant == 1) return true;
+    if (variant == -1) return fused_safe(t, precision);
+    fail(CU_ERR_ILLEGAL_ARGUMENT, "variant must be -1, 0 or 1");
+}
+
+}  // namespace
+}  // namespace cu
+
+using namespace cu;
+
+extern "C" {
+
+int cu_tile_init(cu_tile_desc *t, double lbx, double lby, double rtx, double rty, uint32_t w,
+                 uint32_t h, uint32_t max_iter, double bailout_sq, uint32_t kind, double jcx,
+                 double jcy, uint32_t spp, uint32_t pattern, uint64_t seed) {
+    return guarded([&] {
+        if (!t) fail(CU_ERR_ILLEGAL_ARGUMENT, "tile is NULL");
+        if (w == 0 || h == 0) fail(CU_ERR_ILLEGAL_ARGUMENT, "tile has no pixels");
+        std::memset(t, 0, sizeof *t);
+        t->left = lbx;
+        t->top = rty;
+        t->pixel_w = (rtx - lbx) / (double)w;  // image.size() / gridSize (fractalRendererGeneric.cu:97)
+        t->pixel_h = (rty - lby) / (double)h;
+        const float lf = (float)lbx, bf = (float)lby, rf = (float)rtx, tf = (float)rty;
+        t->left_f = lf;
+        t->top_f = tf;
+        t->pixel_w_f = (rf - lf) / (float)w;
+        t->pixel_h_f = (tf - bf) / (float)h;
+        t->julia_cx = jcx;
+        t->julia_cy = jcy;
+        t->bailout_sq = bailout_sq;
+        t->seed = seed;
+        t->width = w;
+        t->height = h;
+        t->out_pitch = w;
+        t->max_iter = max_iter;
+        t->kind = kind;
+        t->spp = spp;
+        t->pattern = pattern;
+        validate_tile(*t);
+    });
+}
+
+int cu_tile_fused_safe(const cu_tile_desc *t, int precision, int *safe) {
+    return guarded([&] {
+        if (!t || !safe) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        real_name(precision);
+        *safe = fused_safe(*t, precision) ? 1 : 0;
+    });
+}
+
+int cu_engine_create(const char *dir, cu_engine *out) {
+    return guarded([&] {
+        if (!dir || !out) fail(CU_ERR_ILLEGAL_ARGUMENT, "dir/out is NULL");
+        std::string path = std::string(dir) + "/escape_family.hsaco";
+        cu_module m = nullptr;
+        int rc = cu_module_load(path.c_str(), &m);
+        if (rc != CU_OK) fail(rc, cu_last_error());
+        auto *e = new cu_engine_s();
+        e->module = m;
+        check(hipEventCreate(&e->start), "hipEventCreate");
+        check(hipEventCreate(&e->stop), "hipEventCreate");
+        *out = e;
+    });
+}
+
+int cu_engine_destroy(cu_engine e) {
+    return guarded([&] {
+        if (!e) return;
+        ensure_device();
+        if (e->start) (void)hipEventDestroy(e->start);
+        if (e->stop) (void)hipEventDestroy(e->stop);
+        cu_module m = e->module;
+        delete e;
+        if (m && cu_module_unload(m) != CU_OK) fail(CU_ERR_DEVICE, cu_last_error());
+    });
+}
+
+int cu_escape_tile(cu_engine e, int precision, int variant, const cu_tile_desc *tile,
+                   uint32_t *d_counts, double *d_smooth, unsigned long long *d_iters,
+                   cu_stream stream) {
+    return guarded([&] {
+        if (!e || !tile || !d_counts) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        std::lock_guard<std::mutex> lock(e->mutex);
+        validate_tile(*tile);
+        ensure_device();
+        const bool fused = pick_fused(variant, *tile, precision);
+        std::string name = std::string("escapeTile") + real_name(precision) + (fused ? "Fused" : "Exact");
+        cu_function f = function(e, name);
+        cu_tile_desc t = *tile;
+        void *params[] = {&t, &d_counts, &d_smooth, &d_iters};
+        dim3 block(16, 16, 1);
+        dim3 grid((t.width + 15) / 16, (t.height + 15) / 16, 1);
+        launch(e, f, grid, block, params, stream, true);
+    });
+}
+
+int cu_escape_tiles_persistent(cu_engine e, int precision, int variant, const cu_tile_desc *d_tiles,
+                               uint32_t n_tiles, uint64_t total, uint32_t *d_counts,
+                               double *d_smooth, unsigned long long *d_iters,
+                               unsigned long long *d_queue, const cu_tune *tune, cu_stream stream) {
+    return guarded([&] {
+        if (!e || !d_tiles || !d_counts || !d_queue) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        if (n_tiles == 0 || n_tiles > kMaxPersistentTiles)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "n_tiles must be in [1, 1024]");
+        if (variant < -1 || variant > 1) fail(CU_ERR_ILLEGAL_ARGUMENT, "variant must be -1, 0 or 1");
+        if (variant == -1)
+            fail(CU_ERR_ILLEGAL_ARGUMENT,
+                 "the persistent launch takes an explicit variant (check each tile with cu_tile_fused_safe)");
+        std::lock_guard<std::mutex> lock(e->mutex);
+        ensure_device();
+        cu_tune tn{16, 32, 256, 16};
+        if (tune) tn = *tune;
+        if (tn.block_steps != 8 && tn.block_steps != 16 && tn.block_steps != 32)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "block_steps must be 8, 16 or 32");
+        if (tn.waves_per_cu < 4 || tn.waves_per_cu > 32 || tn.waves_per_cu % 4)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "waves_per_cu must be a multiple of 4 in [4, 32]");
+        if (tn.chunk == 0 || tn.chunk > (1u << 20)) fail(CU_ERR_ILLEGAL_ARGUMENT, "chunk must be in [1, 2^20]");
+        if (tn.refill_lanes == 0 || tn.refill_lanes > 64)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "refill_lanes must be in [1, 64]");
+        if (total == 0) return;
+        std::string name = std::string("escapeTilesPersistent") + real_name(precision) +
+                           (variant == 1 ? "Fused" : "Exact") + "K" + std::to_string(tn.block_steps);
+        cu_function f = function(e, name);
+        // clear the queue counter with a kernel (safe under graph capture)
+        cu_function fill = function(e, "fillU32");
+        uint32_t *q32 = (uint32_t *)d_queue;
+        uint32_t zero = 0;
+        uint64_t two = 2;
+        void *fill_params[] = {&q32, &zero, &two};
+        launch(e, fill, dim3(1), dim3(64), fill_params, stream, false);
+        const uint32_t cus = (uint32_t)std::max(1, compute_units());
+        dim3 block(256, 1, 1);
+        dim3 grid(cus * tn.waves_per_cu / 4, 1, 1);
+        uint32_t chunk = tn.chunk, refill = tn.refill_lanes;
+        void *params[] = {&d_tiles, &n_tiles, &total, &d_counts, &d_smooth, &d_iters,
+                          &d_queue, &chunk, &refill};
+        launch(e, f, grid, block, params, stream, true);
+    });
+}
+
+int cu_compose_smooth(cu_engine e, const double *d_smooth, uint32_t smooth_pitch, uint32_t *d_rgba,
+                      uint32_t rgba_pitch, uint32_t w, uint32_t h, const uint32_t *d_palette,
+                      uint32_t palette_length, cu_stream stream) {
+    return guarded([&] {
+        if (!e || !d_smooth || !d_rgba || !d_palette) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        if (palette_length == 0) fail(CU_ERR_ILLEGAL_ARGUMENT, "empty palette");
+        if (smooth_pitch < w || rgba_pitch < w) fail(CU_ERR_ILLEGAL_ARGUMENT, "pitch smaller than width");
+        if (w == 0 || h == 0) return;
+        std::lock_guard<std::mutex> lock(e->mutex);
+        ensure_device();
+        cu_function f = function(e, "composeSmooth");
+        void *params[] = {&d_smooth, &smooth_pitch, &d_rgba, &rgba_pitch, &w, &h, &d_palette,
+                          &palette_length};
+        dim3 block(64, 4, 1);
+        dim3 grid((w + 63) / 64, std::min<uint32_t>((h + 3) / 4, 4096), 1);
+        launch(e, f, grid, block, params, stream, true);
+    });
+}
+
+int cu_engine_last_kernel_ms(cu_engine e, float *ms) {
+    return guarded([&] {
+        if (!e || !ms) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        if (!e->timed) fail(CU_ERR_ILLEGAL_STATE, "no timed launch yet");
+        ensure_device();
+        check(hipEventSynchronize(e->stop), "hipEventSynchronize");
+        check(hipEventElapsedTime(ms, e->start, e->stop), "hipEventElapsedTime");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,21 @@
+// handles.hpp — the structs behind the opaque driver-layer handles.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <map>
+#include <string>
+
+struct cu_function_s {
+    hipFunction_t function;
+    std::string name;
+};
+
+struct cu_module_s {
+    hipModule_t module;
+    std::string path;
+    std::map<std::string, cu_function_s *> functions;  // owned; freed on unload
+};
+
+struct cu_stream_s {
+    hipStream_t stream;
+};

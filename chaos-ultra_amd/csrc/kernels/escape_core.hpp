@@ -1,0 +1,156 @@
+// escape_core.hpp — the quadratic-family orbit z <- z^2 + c for gfx950.
+//
+// Rounding follows the reference's loop (src/main/cuda/fractals/mandelbrot.cu:16-20)
+// with contraction off (the whole tree is built with -ffp-contract=off):
+//     test   x*x + y*y < bailout_sq
+//     x'   = (x*x - y*y) + cx
+//     y'   = ((2*x)*y) + cy
+// The orbit runs in unchecked blocks of K steps.  Once |z|^2 >= R^2 with R >= 2
+// and |c| <= R the orbit stays outside (|z'| >= |z|^2 - |c| >= |z|(|z|-1) >= |z|),
+// and for Mandelbrot (z0 = 0) a c with |c| > R leaves at step 1 and stays out,
+// so one test at the end of a block tells whether the orbit left inside it.  A
+// block whose end test fails is replayed step by step from its saved start,
+// which yields the exact escape step and the |z|^2 of the failing test; inf/NaN
+// produced after the escape inside the discarded block fail the test as well.
+#pragma once
+#include "cu_types.hpp"
+
+#pragma clang fp contract(off)
+
+namespace cu {
+
+// One step.  kFused computes y' as fma(x*y, 2, cy): 2*(x*y) is exact, so this
+// equals ((2x)*y)+cy whenever x*y is zero or at least the smallest normal —
+// one VALU op fewer (6 per step instead of 7).
+template <class Real, bool kFused>
+__device__ __forceinline__ void quad_step(Real &x, Real &y, Real cx, Real cy) {
+    const Real x2 = x * x;
+    const Real y2 = y * y;
+    const Real xn = (x2 - y2) + cx;
+    if constexpr (kFused) {
+        y = __builtin_fma(x * y, (Real)2, cy);
+    } else {
+        y = ((x + x) * y) + cy;
+    }
+    x = xn;
+}
+
+// Result of one orbit.
+template <class Real>
+struct Orbit {
+    uint n;        // iterations executed
+    bool escaped;  // stopped on the bailout test (else n == max_iter)
+    Real mag2;     // x*x + y*y of the failing test when escaped
+};
+
+// The reference loop  while (i < maxIter && x*x+y*y < R^2) { step; ++i; }
+// with unchecked blocks of K steps when `blocks` holds (escape is permanent for
+// these inputs) and a checked tail/replay.
+template <class Real, int K, bool kFused>
+__device__ __forceinline__ Orbit<Real> escape_orbit(Real x, Real y, Real cx, Real cy,
+                                                     uint max_iter, Real bailout_sq,
+                                                     bool blocks) {
+    uint i = 0;
+    if (blocks) {
+        while (max_iter - i >= (uint)K) {
+            const Real sx = x, sy = y;
+#pragma unroll
+            for (int k = 0; k < K; ++k) quad_step<Real, kFused>(x, y, cx, cy);
+            if (!(x * x + y * y < bailout_sq)) {
+                x = sx;
+                y = sy;
+                break;
+            }
+            i += K;
+        }
+    }
+    Orbit<Real> r;
+    r.escaped = false;
+    r.mag2 = 0;
+    while (i < max_iter) {
+        const Real x2 = x * x;
+        const Real y2 = y * y;
+        const Real m = x2 + y2;
+        if (!(m < bailout_sq)) {
+            r.escaped = true;
+            r.mag2 = m;
+            break;
+        }
+        const Real xn = (x2 - y2) + cx;
+        if constexpr (kFused) {
+            y = __builtin_fma(x * y, (Real)2, cy);
+        } else {
+            y = ((x + x) * y) + cy;
+        }
+        x = xn;
+        ++i;
+    }
+    r.n = i;
+    return r;
+}
+
+// Whether unchecked blocks are valid: R >= 2 and, when c is fixed (Julia), |c| <= R.
+__device__ __forceinline__ bool blocks_valid(double bailout_sq, bool fixed_c, double cx,
+                                             double cy) {
+    if (!(bailout_sq >= 4.0)) return false;
+    if (!fixed_c) return true;
+    return cx * cx + cy * cy <= bailout_sq * 0.999;
+}
+
+// n + 1 - log2(ln(mag2) / ln(R^2)) — the smooth (continuous) escape value.
+__device__ __forceinline__ double smooth_value(uint n, double mag2, double bailout_sq) {
+    return (double)n + 1.0 - log2(log(mag2) / log(bailout_sq));
+}
+
+// SplitMix64 finaliser (Steele, Lea, Flood 2014) and the per-sample jitter.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void jitter(uint64_t seed, uint64_t sample_id, double &dx, double &dy) {
+    const uint64_t h1 = splitmix64(seed ^ splitmix64(sample_id));
+    const uint64_t h2 = splitmix64(h1);
+    dx = (double)(h1 >> 11) * 0x1.0p-53;
+    dy = (double)(h2 >> 11) * 0x1.0p-53;
+}
+
+__device__ __forceinline__ void sample_offset(const cu_tile_desc &t, uint px, uint py, uint s,
+                                              double &dx, double &dy) {
+    if (t.pattern == CU_SAMPLE_CENTRE) {
+        dx = 0.5;
+        dy = 0.5;
+    } else if (t.pattern == CU_SAMPLE_CORNER) {
+        dx = 0.0;
+        dy = 0.0;
+    } else {
+        jitter(t.seed, ((uint64_t)py * t.width + px) * t.spp + s, dx, dy);
+    }
+}
+
+// X = left + (px + dx) * pixel_w,  Y = top - (py + dy) * pixel_h
+template <class Real>
+__device__ __forceinline__ void tile_point(const cu_tile_desc &t, uint px, uint py, double dx,
+                                           double dy, Real &X, Real &Y);
+
+template <>
+__device__ __forceinline__ void tile_point<double>(const cu_tile_desc &t, uint px, uint py,
+                                                   double dx, double dy, double &X, double &Y) {
+    X = t.left + ((double)px + dx) * t.pixel_w;
+    Y = t.top - ((double)py + dy) * t.pixel_h;
+}
+
+template <>
+__device__ __forceinline__ void tile_point<float>(const cu_tile_desc &t, uint px, uint py,
+                                                  double dx, double dy, float &X, float &Y) {
+    X = t.left_f + ((float)px + (float)dx) * t.pixel_w_f;
+    Y = t.top_f - ((float)py + (float)dy) * t.pixel_h_f;
+}
+
+// Wave-level helpers (wave64).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ uint lane_id() { return __lane_id(); }
+
+}  // namespace cu

@@ -1,0 +1,298 @@
+// escape_family.hip — planar escape-time kernels of the quadratic family for gfx950.
+//
+// These serve the throughput forms that have no counterpart in the reference:
+// tiles of pixel-centre or seeded-jitter samples with any bailout radius and a
+// smooth value, a persistent launch over many tiles of uneven depth, and a
+// palette compose of the smooth plane.  Mandelbrot and Julia tiles mix freely
+// (per-tile `kind`); the orbit rounding is the reference's (escape_core.hpp).
+//
+// Kernels (all extern "C", bound by name from escape_family.hsaco):
+//   escapeTile{Double,Float}{Exact,Fused}      thread per pixel, one tile
+//   escapeTilesPersistent{Double,Float}{Exact,Fused}
+//                                              lane-refill work queue, many tiles
+//   composeSmooth                              smooth plane -> RGBA8, LDS palette
+//   fillU32                                    device fill (graph-capture safe)
+#include "cu_types.hpp"
+#include "escape_core.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr uint kMaxLdsTiles = 1024;
+constexpr uint kPaletteLds = 4096;
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void add_iterations(unsigned long long *iters, uint64_t v) {
+    v = wave_sum(v);
+    if (iters && cu::lane_id() == 0 && v) atomicAdd(iters, (unsigned long long)v);
+}
+
+// Start sample s of pixel (px, py) of tile t: z0 and c for the tile's kind.
+template <class Real>
+__device__ __forceinline__ void begin_sample(const cu_tile_desc &t, uint px, uint py, uint s,
+                                             Real &x, Real &y, Real &cx, Real &cy) {
+    double dx, dy;
+    cu::sample_offset(t, px, py, s, dx, dy);
+    Real X, Y;
+    cu::tile_point<Real>(t, px, py, dx, dy, X, Y);
+    if (t.kind == CU_FRACTAL_JULIA) {
+        x = X;
+        y = Y;
+        cx = (Real)t.julia_cx;
+        cy = (Real)t.julia_cy;
+    } else {
+        x = 0;
+        y = 0;
+        cx = X;
+        cy = Y;
+    }
+}
+
+__device__ __forceinline__ bool tile_blocks_valid(const cu_tile_desc &t) {
+    return cu::blocks_valid(t.bailout_sq, t.kind == CU_FRACTAL_JULIA, t.julia_cx, t.julia_cy);
+}
+
+// ---------------------------------------------------------------------------
+// thread per pixel over one tile
+// ---------------------------------------------------------------------------
+template <class Real, int K, bool kFused>
+__device__ void escape_tile(const cu_tile_desc &t, uint *counts, double *smooth,
+                            unsigned long long *iters) {
+    const uint px = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint py = blockIdx.y * blockDim.y + threadIdx.y;
+    uint64_t executed = 0;
+    if (px < t.width && py < t.height) {
+        const bool blocks = tile_blocks_valid(t);
+        const Real bsq = (Real)t.bailout_sq;
+        const bool julia = t.kind == CU_FRACTAL_JULIA;
+        uint count_sum = 0;
+        double smooth_sum = 0;
+        for (uint s = 0; s < t.spp; ++s) {
+            Real x, y, cx, cy;
+            begin_sample<Real>(t, px, py, s, x, y, cx, cy);
+            const cu::Orbit<Real> o =
+                cu::escape_orbit<Real, K, kFused>(x, y, cx, cy, t.max_iter, bsq, blocks);
+            executed += o.n;
+            count_sum += o.escaped ? o.n : (julia ? t.max_iter : 0u);
+            smooth_sum += o.escaped ? cu::smooth_value(o.n, (double)o.mag2, t.bailout_sq) : 0.0;
+        }
+        const uint64_t o = t.out_offset + (uint64_t)py * t.out_pitch + px;
+        counts[o] = count_sum / t.spp;
+        if (smooth) smooth[o] = smooth_sum / (double)t.spp;
+    }
+    add_iterations(iters, executed);
+}
+
+// ---------------------------------------------------------------------------
+// persistent lane-refill launch over many tiles
+// ---------------------------------------------------------------------------
+// Every wave keeps 64 independent samples in flight.  Lanes run unchecked
+// blocks of K steps in lock step; a lane whose block fails its end test (or
+// whose remaining budget is below K) becomes pending.  Once `refill` lanes are
+// pending or idle, the pending lanes replay their last block step by step (the
+// exact escape step), finish their sample, start the next sample of the same
+// pixel or write the pixel, and idle lanes take new pixels from a wave-private
+// chunk of the global pixel queue (one atomic per `chunk` pixels).  Pixels of
+// one tile are numbered row-major from the tile's work_offset.  There is no
+// barrier after the tile table is staged, and every wave leaves once the queue
+// is exhausted and its lanes are idle.
+template <class Real, int K, bool kFused>
+__device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, uint n_tiles,
+                                        uint64_t total, uint *counts, double *smooth,
+                                        unsigned long long *iters, unsigned long long *queue,
+                                        uint chunk, uint refill) {
+    __shared__ uint64_t lds_offset[kMaxLdsTiles];
+    for (uint k = threadIdx.x; k < n_tiles; k += blockDim.x) lds_offset[k] = tiles[k].work_offset;
+    __syncthreads();
+
+    const uint lane = cu::lane_id();
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+
+    int mode = 0;          // 0 idle, 1 running unchecked blocks, 2 pending checked work
+    bool replay = false;   // pending lane restarts its failed block from (sx, sy)
+    uint tile = 0, px = 0, py = 0, s = 0, i = 0, count_sum = 0;
+    double smooth_sum = 0;
+    Real x = 0, y = 0, cx = 0, cy = 0, sx = 0, sy = 0, bsq = 4;
+    uint max_iter = 0;
+    uint64_t executed = 0;
+
+    uint64_t cur = 0, end = 0;  // wave-uniform chunk cursor
+    bool exhausted = false;
+
+    for (;;) {
+        // ---- refill idle lanes ----
+        uint64_t idle = cu::ballot(mode == 0);
+        while (idle != 0 && !exhausted) {
+            if (cur >= end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(queue, (unsigned long long)chunk);
+                base = __shfl(base, 0, 64);
+                if (base >= total) {
+                    exhausted = true;
+                    break;
+                }
+                cur = base;
+                end = min((uint64_t)base + chunk, total);
+            }
+            const uint n = (uint)__popcll(idle);
+            const uint64_t avail = end - cur;
+            const uint take = (uint)min((uint64_t)n, avail);
+            const uint rank = (uint)__popcll(idle & lanes_below);
+            if (mode == 0 && rank < take) {
+                const uint64_t item = cur + rank;
+                uint lo = 0, hi = n_tiles - 1;  // last tile with work_offset <= item
+                while (lo < hi) {
+                    const uint mid = (lo + hi + 1) >> 1;
+                    if (lds_offset[mid] <= item) lo = mid; else hi = mid - 1;
+                }
+                tile = lo;
+                const cu_tile_desc &t = tiles[tile];
+                const uint64_t local = item - lds_offset[tile];
+                px = (uint)(local % t.width);
+                py = (uint)(local / t.width);
+                s = 0;
+                count_sum = 0;
+                smooth_sum = 0;
+                max_iter = t.max_iter;
+                bsq = (Real)t.bailout_sq;
+                begin_sample<Real>(t, px, py, 0, x, y, cx, cy);
+                i = 0;
+                replay = false;
+                mode = (tile_blocks_valid(t) && max_iter >= (uint)K) ? 1 : 2;
+            }
+            cur += take;
+            idle = cu::ballot(mode == 0);
+        }
+        if (cu::ballot(mode != 0) == 0) break;
+
+        // ---- unchecked blocks until enough lanes wait ----
+        for (;;) {
+            if (mode == 1) {
+                sx = x;
+                sy = y;
+#pragma unroll
+                for (int k = 0; k < K; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+                if (x * x + y * y < bsq) {
+                    i += K;
+                    if (max_iter - i < (uint)K) mode = 2;
+                } else {
+                    mode = 2;
+                    replay = true;
+                }
+            }
+            const uint64_t run = cu::ballot(mode == 1);
+            const uint waiting = (uint)__popcll(cu::ballot(mode == 2)) +
+                                 (exhausted ? 0u : (uint)__popcll(cu::ballot(mode == 0)));
+            if (run == 0 || waiting >= refill) break;
+        }
+
+        // ---- pending lanes: checked steps, finish the sample ----
+        if (mode == 2) {
+            if (replay) {
+                x = sx;
+                y = sy;
+                replay = false;
+            }
+            bool escaped = false;
+            Real mag2 = 0;
+            while (i < max_iter) {
+                const Real x2 = x * x;
+                const Real y2 = y * y;
+                const Real m = x2 + y2;
+                if (!(m < bsq)) {
+                    escaped = true;
+                    mag2 = m;
+                    break;
+                }
+                const Real xn = (x2 - y2) + cx;
+                if constexpr (kFused) {
+                    y = __builtin_fma(x * y, (Real)2, cy);
+                } else {
+                    y = ((x + x) * y) + cy;
+                }
+                x = xn;
+                ++i;
+            }
+            const cu_tile_desc &t = tiles[tile];
+            executed += i;
+            count_sum += escaped ? i : (t.kind == CU_FRACTAL_JULIA ? max_iter : 0u);
+            smooth_sum += escaped ? cu::smooth_value(i, (double)mag2, t.bailout_sq) : 0.0;
+            ++s;
+            if (s < t.spp) {
+                begin_sample<Real>(t, px, py, s, x, y, cx, cy);
+                i = 0;
+                mode = (tile_blocks_valid(t) && max_iter >= (uint)K) ? 1 : 2;
+            } else {
+                const uint64_t o = t.out_offset + (uint64_t)py * t.out_pitch + px;
+                counts[o] = count_sum / t.spp;
+                if (smooth) smooth[o] = smooth_sum / (double)t.spp;
+                mode = 0;
+            }
+        }
+    }
+    add_iterations(iters, executed);
+}
+
+}  // namespace
+
+#define CU_TILE_KERNELS(NAME, REAL, FUSED)                                                      \
+    extern "C" __global__ __launch_bounds__(256) void escapeTile##NAME(                         \
+        cu_tile_desc tile, uint *counts, double *smooth, unsigned long long *iters) {           \
+        escape_tile<REAL, 16, FUSED>(tile, counts, smooth, iters);                              \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void escapeTilesPersistent##NAME##K8(          \
+        const cu_tile_desc *tiles, uint n_tiles, uint64_t total, uint *counts, double *smooth,  \
+        unsigned long long *iters, unsigned long long *queue, uint chunk, uint refill) {        \
+        escape_tiles_persistent<REAL, 8, FUSED>(tiles, n_tiles, total, counts, smooth, iters,   \
+                                                queue, chunk, refill);                          \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void escapeTilesPersistent##NAME##K16(         \
+        const cu_tile_desc *tiles, uint n_tiles, uint64_t total, uint *counts, double *smooth,  \
+        unsigned long long *iters, unsigned long long *queue, uint chunk, uint refill) {        \
+        escape_tiles_persistent<REAL, 16, FUSED>(tiles, n_tiles, total, counts, smooth, iters,  \
+                                                 queue, chunk, refill);                         \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void escapeTilesPersistent##NAME##K32(         \
+        const cu_tile_desc *tiles, uint n_tiles, uint64_t total, uint *counts, double *smooth,  \
+        unsigned long long *iters, unsigned long long *queue, uint chunk, uint refill) {        \
+        escape_tiles_persistent<REAL, 32, FUSED>(tiles, n_tiles, total, counts, smooth, iters,  \
+                                                 queue, chunk, refill);                         \
+    }
+
+CU_TILE_KERNELS(DoubleExact, double, false)
+CU_TILE_KERNELS(DoubleFused, double, true)
+CU_TILE_KERNELS(FloatExact, float, false)
+CU_TILE_KERNELS(FloatFused, float, true)
+
+// Palette compose of a smooth plane: idx = L - (round(v) mod L) - 1 (the
+// colorize of mandelbrot.cu:27-35 applied to the smooth value).
+extern "C" __global__ __launch_bounds__(256) void composeSmooth(
+    const double *smooth, uint smooth_pitch, uint *rgba, uint rgba_pitch, uint width,
+    uint height, const uint *palette, uint palette_length) {
+    __shared__ uint lds_palette[kPaletteLds];
+    const bool staged = palette_length <= kPaletteLds;
+    const uint nthreads = blockDim.x * blockDim.y;
+    const uint tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if (staged)
+        for (uint k = tid; k < palette_length; k += nthreads) lds_palette[k] = palette[k];
+    __syncthreads();
+    const uint *pal = staged ? lds_palette : palette;
+    for (uint y = blockIdx.y * blockDim.y + threadIdx.y; y < height; y += gridDim.y * blockDim.y)
+        for (uint x = blockIdx.x * blockDim.x + threadIdx.x; x < width;
+             x += gridDim.x * blockDim.x) {
+            const uint vi = (uint)round(smooth[(size_t)y * smooth_pitch + x]);
+            rgba[(size_t)y * rgba_pitch + x] = pal[palette_length - (vi % palette_length) - 1];
+        }
+}
+
+extern "C" __global__ __launch_bounds__(256) void fillU32(uint *p, uint value, uint64_t n) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (uint64_t)gridDim.x * blockDim.x)
+        p[k] = value;
+}

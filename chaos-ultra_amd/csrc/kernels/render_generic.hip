@@ -1,0 +1,447 @@
+// render_generic.hip — the kernels every fractal module exports, for gfx950.
+//
+// Built once per fractal with -DCU_FRACTAL_HEADER="fractals/<name>.hip" naming
+// the fractal's file (it defines `struct Fractal` and debugFractal()), into
+// <name>.hsaco, which the host
+// loads with hipModuleLoad.  Kernel names, parameter lists and the module
+// global VISUALIZE_SAMPLE_COUNT are the module ABI of
+// src/main/cuda/fractalRendererGeneric.cu, so a host written against the
+// reference binds them unchanged; the one difference is compose, whose two GL
+// surface handles are plain device pointers here (a linear RGBA8 image and the
+// palette), each still one 8-byte argument.
+//
+// Launch geometry is the reference's: blocks of 32x32 threads over
+// ceil(W/32) x ceil(H/32) blocks.  A wave64 covers two of the reference's
+// 32-lane warps, i.e. two 8x4-pixel rectangles side by side; each half-wave is
+// one vote group of the adaptive supersampling, exactly the 8x4 rectangles the
+// reference votes over.  No lane returns early: out-of-image lanes stay in the
+// wave with no work, so every ballot runs in wave-uniform control flow and the
+// vote membership is explicit (same half-wave, same call site, still sampling).
+#include "cu_types.hpp"
+#include "escape_core.hpp"
+
+#include <cfloat>
+
+#ifndef CU_FRACTAL_HEADER
+#error "build with -DCU_FRACTAL_HEADER=\"fractals/<name>.hip\""
+#endif
+#include CU_FRACTAL_HEADER
+
+#pragma clang fp contract(off)
+
+__constant__ bool VISUALIZE_SAMPLE_COUNT = false;
+
+namespace {
+
+constexpr uint kBlockSteps = 16;       // unchecked iterations per block
+constexpr float kPiF = 3.14159265358979f;
+
+// Flag bits (fractalRendererGeneric.cu:157-161)
+constexpr uint kAdaptiveFlag = 1u << 0;
+constexpr uint kFoveationFlag = 1u << 2;
+constexpr uint kReuseFlag = 1u << 3;
+constexpr uint kZoomingFlag = 1u << 4;
+constexpr uint kZoomingInFlag = 1u << 5;
+
+// getImageIndexes (fractalRendererGeneric.cu:36-53): thread -> pixel so that
+// each 32-thread group covers an 8x4 rectangle of the 32x32 block tile.
+__device__ __forceinline__ Point<uint> image_indexes() {
+    const uint t = threadIdx.x + threadIdx.y * 32u;
+    const uint warp = t >> 5, lane = t & 31u;
+    Point<uint> p;
+    p.x = blockIdx.x * 32u + (warp & 3u) * 8u + (lane >> 4) * 4u + (lane & 3u);
+    p.y = blockIdx.y * 32u + (warp >> 2) * 4u + ((lane >> 2) & 3u);
+    return p;
+}
+
+__device__ __forceinline__ pixel_info_t *pixel_at(pixel_info_t *base, long pitch, uint x, uint y) {
+    return (pixel_info_t *)((char *)base + (long)y * pitch) + x;
+}
+__device__ __forceinline__ const pixel_info_t *pixel_at(const pixel_info_t *base, long pitch,
+                                                        uint x, uint y) {
+    return (const pixel_info_t *)((const char *)base + (long)y * pitch) + x;
+}
+
+// One 16-byte store with the padding bytes cleared.
+__device__ __forceinline__ void store_pixel(pixel_info_t *dst, const pixel_info_t &p) {
+    uint4 v;
+    v.x = __float_as_uint(p.value);
+    v.y = __float_as_uint(p.weight);
+    v.z = p.isReused ? 1u : 0u;
+    v.w = __float_as_uint(p.weightOfNewSamples);
+    *reinterpret_cast<uint4 *>(dst) = v;
+}
+
+__device__ __forceinline__ pixel_info_t load_pixel(const pixel_info_t *src) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(src);
+    pixel_info_t p;
+    p.value = __uint_as_float(v.x);
+    p.weight = __uint_as_float(v.y);
+    p.isReused = (v.z & 0xffu) != 0;
+    p.weightOfNewSamples = __uint_as_float(v.w);
+    return p;
+}
+
+// computeFractal<Real> of the module, bailout 4 (the reference's constant).
+template <class Real>
+__device__ __forceinline__ float compute_fractal(uint max_iter, Real X, Real Y, bool blocks) {
+    Real zx, zy, cx, cy;
+    Fractal::start<Real>(X, Y, zx, zy, cx, cy);
+    const cu::Orbit<Real> o =
+        cu::escape_orbit<Real, kBlockSteps, false>(zx, zy, cx, cy, max_iter, (Real)4, blocks);
+    return Fractal::value(o.n, max_iter);
+}
+
+// computeDispersion (fractalRendererGeneric.cu:24-34) over the stored samples;
+// the array lives in registers, so it is indexed with unrolled selects.
+__device__ __forceinline__ float dispersion(const float (&samples)[kAdaptiveThreshold], uint n,
+                                            float mean) {
+    float variance = 0;
+#pragma unroll
+    for (uint j = 0; j < kAdaptiveThreshold; ++j)
+        if (j < n) variance += (samples[j] - mean) * (samples[j] - mean);
+    variance /= (n - 1);
+    return variance / mean;
+}
+
+// sampleTheFractal<Real> (fractalRendererGeneric.cu:85-155) for the lanes with
+// site != 0; lanes sharing a half-wave and a site vote together.  Must be called
+// by the whole wave (the loop and the ballots are wave-uniform).
+template <class Real>
+__device__ uint sample_the_fractal(int site, Point<uint> pixel, Point<uint> grid,
+                                   Rectangle<Real> image, uint max_iter, float &sample_count_f,
+                                   bool adaptive) {
+    const uint64_t half = (cu::lane_id() < 32) ? 0x00000000ffffffffull : 0xffffffff00000000ull;
+    const uint64_t site1 = cu::ballot(site == 1), site2 = cu::ballot(site == 2);
+    const uint64_t group = half & (site == 1 ? site1 : (site == 2 ? site2 : 0ull));
+    const bool blocks = Fractal::blocks_valid(4.0);
+
+    bool in_loop = site != 0;
+    uint sample_count = 0, escape_sum = 0, result = 0;
+    float samples[kAdaptiveThreshold];
+#pragma unroll
+    for (uint j = 0; j < kAdaptiveThreshold; ++j) samples[j] = 0;
+    if (in_loop) {
+        if (sample_count_f < 1) {  // :87-90
+            sample_count_f = 0;
+            in_loop = false;
+        } else {
+            sample_count = min(kMaxSuperSampling, (uint)roundf(sample_count_f));
+        }
+    }
+    // pixelSize = image.size() / gridSize.cast<Real>()  (:97)
+    const Real psx = (image.right_top.x - image.left_bottom.x) / (Real)grid.x;
+    const Real psy = (image.right_top.y - image.left_bottom.y) / (Real)grid.y;
+    const uint skip_first = 2;
+
+    for (uint i = 0; __any(in_loop); ++i) {
+        if (in_loop) {
+            Real dx, dy;
+            if (i <= skip_first) {
+                dx = dy = (Real)i / (Real)(skip_first + 1);
+            } else {
+                const float per_row_f = sqrtf(sample_count_f - skip_first);
+                const uint per_row_i = (uint)roundf(per_row_f);
+                dx = (float)((i - skip_first) % per_row_i) / per_row_f;
+                dy = (float)((i - skip_first) / per_row_i) / per_row_f;
+            }
+            const Real X = image.left_bottom.x + ((Real)pixel.x + dx) * psx;
+            const Real Y = image.right_top.y - ((Real)pixel.y + dy) * psy;
+            const uint escape_time = (uint)compute_fractal<Real>(max_iter, X, Y, blocks);
+            escape_sum += escape_time;
+#pragma unroll
+            for (uint j = 0; j < kAdaptiveThreshold; ++j)
+                if (j == i) samples[j] = (float)escape_time;
+        }
+        const bool part = in_loop && ((adaptive && (i > 0 && i < kAdaptiveThreshold)) ||
+                                      (i == sample_count / 2));
+        float disp = 0;
+        if (part) {
+            const float mean = escape_sum / (i + 1);
+            disp = dispersion(samples, i, mean);
+        }
+        const uint64_t P = cu::ballot(part) & group;
+        const bool v1 =
+            (cu::ballot(part && fabsf(samples[0] - samples[1]) < FLT_EPSILON) & group) == P;
+        const bool v2 = (cu::ballot(part && disp < 0.01f) & group) == P;
+        const bool v3 = (cu::ballot(part && disp <= 1.f) & group) == P;
+        if (part) {
+            if (i == 1 && v1) {
+                sample_count = i + 1;
+            } else if (v2) {
+                sample_count = i + 1;
+            } else if (v3 && i >= sample_count / 2) {
+                sample_count = i + 1;
+            }
+        }
+        if (in_loop && !(i + 1 < sample_count)) {
+            in_loop = false;
+            result = escape_sum / sample_count;
+            sample_count_f = (float)sample_count;
+        }
+    }
+    return result;
+}
+
+template <class Real>
+__device__ void render_main(pixel_info_t *out, long out_pitch, Point<uint> size,
+                            Rectangle<Real> image, uint max_iter, float max_ss, uint flags) {
+    const Point<uint> idx = image_indexes();
+    const bool inside = idx.x < size.x && idx.y < size.y;
+    float sample_count = max_ss;
+    const uint r = sample_the_fractal<Real>(inside ? 1 : 0, idx, size, image, max_iter,
+                                            sample_count, (flags & kAdaptiveFlag) != 0);
+    if (inside) {
+        pixel_info_t p;
+        p.value = (float)r;
+        p.weight = sample_count;
+        p.isReused = false;
+        p.weightOfNewSamples = 0;
+        store_pixel(pixel_at(out, out_pitch, idx.x, idx.y), p);
+    }
+}
+
+// getWarpingOriginOfSampleReuse<Real> (fractalRendererGeneric.cu:194-203)
+template <class Real>
+__device__ __forceinline__ Point<float> warping_origin(Point<uint> p, Point<uint> grid,
+                                                       Rectangle<Real> cur, Rectangle<Real> old) {
+    p.y = grid.y - p.y;
+    const Real plane_x =
+        (Real)p.x / (Real)grid.x * (cur.right_top.x - cur.left_bottom.x) + cur.left_bottom.x;
+    const Real plane_y =
+        (Real)p.y / (Real)grid.y * (cur.right_top.y - cur.left_bottom.y) + cur.left_bottom.y;
+    const Real rel_x = (plane_x - old.left_bottom.x) / (old.right_top.x - old.left_bottom.x);
+    const Real rel_y = (plane_y - old.left_bottom.y) / (old.right_top.y - old.left_bottom.y);
+    Point<float> o;
+    o.x = (float)rel_x * (float)grid.x;
+    o.y = (float)rel_y * (float)grid.y;
+    o.y = grid.y - o.y;
+    return o;
+}
+
+// readFromArrayUsingFiltering<float> (fractalRendererGeneric.cu:259-302)
+__device__ __forceinline__ pixel_info_t read_filtered(const pixel_info_t *arr, long pitch,
+                                                      Point<float> c) {
+    const uint i = (uint)floorf(c.x);
+    const uint j = (uint)floorf(c.y);
+    const float alpha = c.x - i;
+    const float beta = c.y - j;
+    const pixel_info_t t00 = load_pixel(pixel_at(arr, pitch, i, j));
+    const pixel_info_t t10 = load_pixel(pixel_at(arr, pitch, i + 1, j));
+    const pixel_info_t t01 = load_pixel(pixel_at(arr, pitch, i, j + 1));
+    const pixel_info_t t11 = load_pixel(pixel_at(arr, pitch, i + 1, j + 1));
+    pixel_info_t r;
+    r.value = (1 - alpha) * (1 - beta) * t00.value + alpha * (1 - beta) * t10.value +
+              (1 - alpha) * beta * t01.value + alpha * beta * t11.value;
+    r.weight = (1 - alpha) * (1 - beta) * t00.weight + alpha * (1 - beta) * t10.weight +
+               (1 - alpha) * beta * t01.weight + alpha * beta * t11.weight;
+    r.isReused = false;
+    r.weightOfNewSamples = 0;
+    return r;
+}
+
+// getFoveationAdvisedSampleCount + linearMapping (fractalRendererGeneric.cu:209-257).
+// The arctangent is taken in double and rounded once to float.
+__device__ __forceinline__ void foveation(Point<uint> pixel, Point<uint> focus, float max_ss,
+                                          float &advised, bool &inside) {
+    const uint nx = pixel.x - pixel.x % kVoteGroupW, ny = pixel.y - pixel.y % kVoteGroupH;
+    const float foveal = max_ss >= 1 ? 5.5 : 5.5 * (double)max_ss;
+    const float peripheral = 60;
+    const float screen_distance = 60;
+    const float pixel_width_cm = 0.02652;
+    const float ddx = (float)focus.x - (float)nx, ddy = (float)focus.y - (float)ny;
+    const float focus_distance = sqrtf(ddx * ddx + ddy * ddy) * pixel_width_cm;
+    const float angle = (float)atan((double)(focus_distance / screen_distance)) * 180 / kPiF;
+    const float a = 1, b = 0, r = foveal, s = peripheral;
+    const float k = (b - a) / (s - r);
+    const float q = (s * a - b * r) / (s - r);
+    const float lm = k * angle + q;
+    const float rel = (float)fmin(1.0, (double)lm);
+    advised = max_ss * rel;
+    inside = false;
+    if (angle <= foveal) {
+        advised = (float)fmax(1.0, (double)advised);
+        inside = true;
+    }
+}
+
+template <class Real>
+__device__ void render_advanced(pixel_info_t *out, long out_pitch, Point<uint> size,
+                                Rectangle<Real> image, uint max_iter, float max_ss, uint flags,
+                                Rectangle<Real> image_reused, const pixel_info_t *in,
+                                long in_pitch, Point<uint> focus) {
+    const Point<uint> idx = image_indexes();
+    const bool in_image = idx.x < size.x && idx.y < size.y;
+
+    float advised = max_ss;
+    bool in_focus = false;
+    if (in_image && (flags & kFoveationFlag) && (flags & kZoomingFlag) && (flags & kZoomingInFlag))
+        foveation(idx, focus, max_ss, advised, in_focus);
+
+    bool reusing = false;
+    pixel_info_t reused;
+    reused.value = 0;
+    reused.weight = 0;
+    reused.isReused = false;
+    reused.weightOfNewSamples = 0;
+    // (a 1-pixel-wide or -high image would wrap size-2 below and read outside the
+    // previous frame; such images never reuse)
+    if (in_image && (flags & kReuseFlag) && size.x >= 2 && size.y >= 2) {
+        const Point<float> origin = warping_origin<Real>(idx, size, image, image_reused);
+        const int ox = (int)roundf(origin.x), oy = (int)roundf(origin.y);
+        if (ox < 2 || (uint)ox >= size.x - 2 || oy < 2 || (uint)oy >= size.y - 2) {
+            reusing = false;
+        } else {
+            reused = read_filtered(in, in_pitch, origin);
+            reusing = !((double)reused.weight < 0.1);
+        }
+    }
+
+    float sample_count = advised;
+    pixel_info_t result;
+    result.value = 0;
+    result.weight = 0;
+    result.isReused = false;
+    result.weightOfNewSamples = 0;
+    int site = 0;
+    if (in_image) {
+        if (reusing) {
+            result = reused;
+            result.isReused = true;
+            result.weightOfNewSamples = 0;
+            if ((flags & kZoomingInFlag) && in_focus) site = 1;
+        } else {
+            if (sample_count < 1) sample_count = 1;
+            site = 2;
+        }
+    }
+    const uint r = sample_the_fractal<Real>(site, idx, size, image, max_iter, sample_count,
+                                            (flags & kAdaptiveFlag) != 0);
+    if (site == 1) {  // :349-356
+        const float samples = (float)r;
+        reused.weight = (float)((double)reused.weight * 0.75);
+        result.weightOfNewSamples = sample_count;
+        result.weight = reused.weight + sample_count;
+        result.value = (reused.value * reused.weight + samples * sample_count) / result.weight;
+    } else if (site == 2) {  // :358-362
+        result.value = (float)r;
+        result.weight = sample_count;
+    }
+    if (in_image) store_pixel(pixel_at(out, out_pitch, idx.x, idx.y), result);
+}
+
+// colorizeSampleCount (fractalRendererGeneric.cu:67-77); the float -> char
+// conversion of 255*rel is truncation followed by the low byte.
+__device__ __forceinline__ uint colorize_sample_count(uint sample_count, uint full) {
+    sample_count = min(sample_count, full);
+    const float rel = sample_count / (float)full;
+    const uint v = (uint)(int)(255 * rel) & 0xffu;
+    return v | (v << 8) | (v << 16) | (0xffu << 24);
+}
+
+}  // namespace
+
+extern "C" __global__ void init() {}
+
+extern "C" __global__ __launch_bounds__(1024) void fractalRenderMainFloat(
+    pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<float> image,
+    uint maxIterations, float maxSuperSampling, uint flags) {
+    render_main<float>(output, outputPitch, outputSize, image, maxIterations, maxSuperSampling,
+                       flags);
+}
+
+extern "C" __global__ __launch_bounds__(1024) void fractalRenderMainDouble(
+    pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<double> image,
+    uint maxIterations, float maxSuperSampling, uint flags) {
+    render_main<double>(output, outputPitch, outputSize, image, maxIterations, maxSuperSampling,
+                        flags);
+}
+
+extern "C" __global__ __launch_bounds__(1024) void fractalRenderAdvancedFloat(
+    pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<float> image,
+    uint maxIterations, float maxSuperSampling, uint flags, Rectangle<float> imageReused,
+    const pixel_info_t *input, long inputPitch, Point<uint> focus) {
+    render_advanced<float>(output, outputPitch, outputSize, image, maxIterations,
+                           maxSuperSampling, flags, imageReused, input, inputPitch, focus);
+}
+
+extern "C" __global__ __launch_bounds__(1024) void fractalRenderAdvancedDouble(
+    pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<double> image,
+    uint maxIterations, float maxSuperSampling, uint flags, Rectangle<double> imageReused,
+    const pixel_info_t *input, long inputPitch, Point<uint> focus) {
+    render_advanced<double>(output, outputPitch, outputSize, image, maxIterations,
+                            maxSuperSampling, flags, imageReused, input, inputPitch, focus);
+}
+
+// compose (fractalRendererGeneric.cu:455-475).  The palette is staged in LDS
+// (up to kPaletteLds entries; longer palettes are read from global memory) and
+// the block strides over the image, so any grid covers it; the reference's
+// ceil(W/32) x ceil(H/32) grid of 32x32 blocks writes one pixel per thread.
+constexpr uint kPaletteLds = 4096;
+
+extern "C" __global__ __launch_bounds__(1024) void compose(
+    const pixel_info_t *inputMain, long inputMainPitch, const pixel_info_t *inputBcg,
+    long inputBcgPitch, uint *output, uint width, uint height, const uint *palette,
+    uint paletteLength, float maxSuperSampling) {
+    (void)inputBcg;
+    (void)inputBcgPitch;
+    __shared__ uint lds_palette[kPaletteLds];
+    const bool staged = paletteLength <= kPaletteLds;
+    const uint nthreads = blockDim.x * blockDim.y;
+    const uint tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if (staged)
+        for (uint k = tid; k < paletteLength; k += nthreads) lds_palette[k] = palette[k];
+    __syncthreads();
+    const uint *pal = staged ? lds_palette : palette;
+    const bool visualise = VISUALIZE_SAMPLE_COUNT;
+    const uint full = (uint)fmax(1.0, (double)maxSuperSampling);
+    for (uint y = blockIdx.y * blockDim.y + threadIdx.y; y < height; y += gridDim.y * blockDim.y)
+        for (uint x = blockIdx.x * blockDim.x + threadIdx.x; x < width;
+             x += gridDim.x * blockDim.x) {
+            const pixel_info_t p = load_pixel(pixel_at(inputMain, inputMainPitch, x, y));
+            uint color;
+            if (visualise) {
+                color = colorize_sample_count((uint)p.weight, full);
+                if (p.isReused) color = colorize_sample_count((uint)p.weightOfNewSamples, full);
+            } else {
+                // colorize (mandelbrot.cu:27-35)
+                const uint vi = (uint)roundf(p.value);
+                color = pal[paletteLength - (vi % paletteLength) - 1];
+            }
+            output[(size_t)y * width + x] = color;
+        }
+}
+
+// fractalRenderUnderSampled (fractalRendererGeneric.cu:478-503): never launched
+// by the reference's host; kept with its CUDA parameter list.  Thread (X, Y)
+// computes the pixel (X*level, Y*level) in FP32 and writes it once.
+extern "C" __global__ void fractalRenderUnderSampled(pixel_info_t *output, long outputPitch,
+                                                     uint width, uint height, float left_bottom_x,
+                                                     float left_bottom_y, float right_top_x,
+                                                     float right_top_y, uint maxIterations,
+                                                     uint underSamplingLevel) {
+    // a level of 0 or above the image size would wrap `width - level` and write
+    // outside the image; nothing is written then
+    if (underSamplingLevel == 0 || underSamplingLevel > width || underSamplingLevel > height) return;
+    const uint idx_x = (blockDim.x * blockIdx.x + threadIdx.x) * underSamplingLevel;
+    const uint idx_y = (blockDim.y * blockIdx.y + threadIdx.y) * underSamplingLevel;
+    if (idx_x >= width - underSamplingLevel || idx_y >= height - underSamplingLevel) return;
+    const float pw = (right_top_x - left_bottom_x) / (float)width;
+    const float ph = (right_top_y - left_bottom_y) / (float)height;
+    const float cx = left_bottom_x + idx_x * pw;
+    const float cy = right_top_y - idx_y * ph;
+    const uint e = (uint)compute_fractal<float>(maxIterations, cx, cy, Fractal::blocks_valid(4.0));
+    pixel_info_t p;
+    p.value = (float)e;
+    p.weight = 1 / (float)underSamplingLevel;
+    p.isReused = false;
+    p.weightOfNewSamples = 0;
+    store_pixel(pixel_at(output, outputPitch, idx_x, idx_y), p);
+}
+
+// debug (fractalRendererGeneric.cu:505-513): thread (0,0) calls debugFractal().
+extern "C" __global__ void debug() {
+    const uint x = blockDim.x * blockIdx.x + threadIdx.x;
+    const uint y = blockDim.y * blockIdx.y + threadIdx.y;
+    if (x == 0 && y == 0) debugFractal();
+}

@@ -1,0 +1,298 @@
+/*
+ * chaosultra.h — C ABI of the MI355X escape-time render path.
+ *
+ * Three layers, all plain C (pointers, sizes, status codes):
+ *
+ *  1. Driver layer.  The calls the reference's JCuda binding makes on this path,
+ *     one entry point each (context, module, function, global, pitched memory,
+ *     launch, synchronize).  A Java/JNI or Panama shim binds these where the
+ *     reference bound jcuda.driver.JCudaDriver / jcuda.runtime.JCuda.
+ *  2. Renderer layer.  The reference's plugin surface (FractalRendererProvider,
+ *     FractalRenderer, RenderingModel) implemented natively in C++ on top of
+ *     layer 1 and exported as handles, so a thin Java shim can forward each
+ *     interface method to one call.
+ *  3. Escape-family layer.  Throughput forms with no counterpart in the
+ *     reference: planar escape-time tiles (pixel centres or seeded jitter,
+ *     any bailout radius, smooth value), the persistent multi-tile launch,
+ *     palette compose of smooth values, and the FP32 perturbation path.
+ *
+ * Status codes map 1:1 onto the reference's exception classes; the message of
+ * the last failing call on the calling thread is returned by cu_last_error().
+ * Every call is synchronous unless it takes a stream argument.
+ */
+#ifndef CHAOSULTRA_H
+#define CHAOSULTRA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* status codes                                                              */
+/* ------------------------------------------------------------------------- */
+enum cu_status {
+    CU_OK = 0,
+    CU_ERR_ILLEGAL_ARGUMENT = 1, /* java.lang.IllegalArgumentException        */
+    CU_ERR_INITIALIZATION = 2,   /* cudarenderer.CudaInitializationException   */
+    CU_ERR_RENDERER = 3,         /* rendering.FractalRendererException         */
+    CU_ERR_ILLEGAL_STATE = 4,    /* java.lang.IllegalStateException            */
+    CU_ERR_DEVICE = 5            /* jcuda.CudaException (any other HIP error)  */
+};
+
+/* Message of the last non-CU_OK return on this thread ("" if none). */
+const char *cu_last_error(void);
+
+/* ------------------------------------------------------------------------- */
+/* 1. driver layer                                                           */
+/* ------------------------------------------------------------------------- */
+
+/* Replaces CudaHelpers.cudaInit (cudarenderer/CudaHelpers.java:27-45):
+ * cuInit + cuDeviceGet(device) + cuCtxCreate.  Idempotent. */
+int cu_init(int device);
+/* Number of visible devices; 0 (and CU_OK) on a machine without a GPU. */
+int cu_device_count(int *count);
+
+typedef struct cu_module_s *cu_module;
+typedef struct cu_function_s *cu_function;
+
+/* cuModuleLoad (cudarenderer/FractalRenderingModule.java:73-88).  A missing
+ * file is CU_ERR_ILLEGAL_ARGUMENT and is reported before any device call;
+ * other load failures are CU_ERR_INITIALIZATION. */
+int cu_module_load(const char *path, cu_module *out);
+/* cuModuleUnload (cudarenderer/FractalRenderingModule.java:140-147). */
+int cu_module_unload(cu_module module);
+/* cuModuleGetFunction (cudarenderer/CudaKernel.java:28-43); an unknown name is
+ * CU_ERR_ILLEGAL_ARGUMENT, as the reference's CUDA_ERROR_NOT_FOUND mapping. */
+int cu_module_get_function(cu_module module, const char *name, cu_function *out);
+/* cuModuleGetGlobal (cudarenderer/FractalRenderingModule.java:163-169). */
+int cu_module_get_global(cu_module module, const char *name, void **dptr, size_t *bytes);
+/* writeToConstantMemory (cudarenderer/FractalRenderingModule.java:171-227):
+ * fails with CU_ERR_ILLEGAL_ARGUMENT if the symbol is smaller than `bytes`;
+ * otherwise copies min(symbol size, bytes) bytes and zero-fills the rest of the
+ * symbol (the reference copies the full symbol size from a host array of the
+ * value's size). */
+int cu_module_write_global(cu_module module, const char *name, const void *value, size_t bytes);
+
+/* cuMemAlloc / cuMemAllocPitch (cudarenderer/DeviceMemoryDoubleBuffer2D.java:116-146) */
+int cu_mem_alloc(void **dptr, size_t bytes);
+int cu_mem_alloc_pitch(void **dptr, size_t *pitch, size_t width_bytes, size_t height,
+                       unsigned element_bytes);
+int cu_mem_free(void *dptr);
+int cu_memcpy_htod(void *dst, const void *src, size_t bytes);
+int cu_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int cu_memcpy_dtod(void *dst, const void *src, size_t bytes);
+/* cuMemcpy2D device→host (cudarenderer/CudaFractalRenderer.java:119-134) */
+int cu_memcpy_2d_dtoh(void *dst, size_t dst_pitch, const void *src, size_t src_pitch,
+                      size_t width_bytes, size_t height);
+/* Synchronous fill of `count` 32-bit words. */
+int cu_memset_u32(void *dptr, uint32_t value, size_t count);
+
+/* cuLaunchKernel on the null stream (cudarenderer/CudaFractalRenderer.java:371-378);
+ * `params` is the JCuda-style array of pointers to each argument. */
+int cu_launch_kernel(cu_function f, unsigned grid_x, unsigned grid_y, unsigned grid_z,
+                     unsigned block_x, unsigned block_y, unsigned block_z,
+                     unsigned shared_bytes, void **params);
+/* cuCtxSynchronize */
+int cu_ctx_synchronize(void);
+
+/* Streams and events (HIP-native additions used by the batched request loop). */
+typedef struct cu_stream_s *cu_stream;
+int cu_stream_create(cu_stream *out);
+int cu_stream_destroy(cu_stream s);
+int cu_stream_synchronize(cu_stream s);
+/* Non-blocking completion query: *done = 1 when all work on s has finished. */
+int cu_stream_query(cu_stream s, int *done);
+
+/* ------------------------------------------------------------------------- */
+/* 2. renderer layer (FractalRenderer / FractalRendererProvider)             */
+/* ------------------------------------------------------------------------- */
+
+/* FractalRendererState (rendering/FractalRendererState.java) */
+enum { CU_STATE_NOT_INITIALIZED = 0, CU_STATE_READY_TO_RENDER = 1 };
+/* FloatPrecision (util/FloatPrecision.java) */
+enum { CU_PRECISION_SINGLE = 0, CU_PRECISION_DOUBLE = 1, CU_PRECISION_TOO_BIG = 2 };
+
+/* Flag bits of the kernel flag word (fractalRendererGeneric.cu:157-161,
+ * cudarenderer/KernelMain.java:12-13, cudarenderer/KernelAdvanced.java:17-20). */
+enum {
+    CU_FLAG_ADAPTIVE_SS = 1u << 0,
+    CU_FLAG_VISUALISE_SAMPLE_COUNT = 1u << 1,
+    CU_FLAG_FOVEATION = 1u << 2,
+    CU_FLAG_SAMPLE_REUSE = 1u << 3,
+    CU_FLAG_ZOOMING = 1u << 4,
+    CU_FLAG_ZOOMING_IN = 1u << 5
+};
+
+/* RenderingModel (rendering/model/RenderingModel.java:6-20 and the seven
+ * interfaces it extends) as a plain struct.  Fields marked in/out are written
+ * back by the renderer as the Java model setters would be. */
+typedef struct cu_rendering_model {
+    double left_bottom_x, left_bottom_y, right_top_x, right_top_y; /* PlaneSegment */
+    int32_t max_iterations;                                        /* IterationLimitModel */
+    float max_super_sampling;                                      /* SuperSamplingModel */
+    int32_t use_adaptive_super_sampling;
+    int32_t visualise_sample_count;
+    int32_t use_foveated_rendering;                                /* FoveatedRenderingModel */
+    int32_t mouse_focus_x, mouse_focus_y;
+    int32_t zooming, zooming_in;
+    int32_t use_sample_reuse;                                      /* SampleReuseModel */
+    int32_t sample_reuse_cache_dirty;                              /* in/out */
+    int32_t floating_point_precision;                              /* out: CU_PRECISION_* */
+    int32_t error_count;   /* out: PublicErrorLogger.logError calls made by this render */
+    char last_error[256];  /* out: text of the last logged error                      */
+    char fractal_custom_params[256]; /* DefaultFractalModel */
+} cu_rendering_model;
+
+typedef int32_t cu_provider; /* handle, > 0 */
+typedef int32_t cu_renderer; /* handle, > 0; stale handles are CU_ERR_ILLEGAL_STATE */
+
+/* CudaFractalRendererProvider() — registers the fractal modules found as
+ * <kernels_dir>/<name>.hsaco (the reference's cudaKernelsDir property). */
+int cu_provider_create(const char *kernels_dir, cu_provider *out);
+int cu_provider_destroy(cu_provider p);
+/* getAvailableFractals(): comma-separated, sorted. */
+int cu_provider_available_fractals(cu_provider p, char *buf, size_t buf_len);
+int cu_provider_get_default_renderer(cu_provider p, cu_renderer *out);
+int cu_provider_get_renderer(cu_provider p, const char *fractal_name, int force_reload,
+                             cu_renderer *out);
+
+/* FractalRenderer (rendering/FractalRenderer.java:29-76).  The GL textures of
+ * GLParams are replaced by an RGBA8 device image of width×height and a palette
+ * of palette_length RGBA words (R in the least significant byte). */
+int cu_renderer_initialize_rendering(cu_renderer r, int width, int height,
+                                     const uint32_t *palette, int palette_length);
+int cu_renderer_free_rendering_resources(cu_renderer r);
+int cu_renderer_get_state(cu_renderer r, int *state);
+int cu_renderer_get_width(cu_renderer r, int *width);
+int cu_renderer_get_height(cu_renderer r, int *height);
+int cu_renderer_launch_debug_kernel(cu_renderer r);
+int cu_renderer_render_fast(cu_renderer r, cu_rendering_model *model);
+int cu_renderer_render_quality(cu_renderer r, cu_rendering_model *model);
+int cu_renderer_close(cu_renderer r);
+int cu_renderer_set_fractal_custom_params(cu_renderer r, const char *text);
+int cu_renderer_get_fractal_name(cu_renderer r, char *buf, size_t buf_len);
+int cu_renderer_supply_default_values(cu_renderer r, cu_rendering_model *model, int canvas_width,
+                                      int canvas_height);
+/* Read back the composed RGBA8 image (width*height words, row-major). */
+int cu_renderer_read_output(cu_renderer r, uint32_t *rgba, size_t count);
+/* Read back the primary (1) or secondary (0) pixel_info_t buffer, rows packed
+ * (width*16 bytes each). */
+int cu_renderer_read_pixel_info(cu_renderer r, int primary, void *dst, size_t bytes);
+/* Debug mode (-Ddebug=true): launch errors are returned instead of only logged. */
+int cu_set_debug_mode(int enabled);
+
+/* ---- host-side model logic (no device needed) ---- */
+
+/* Model.setPlaneSegmentFromCenter (rendering/Model.java:247-256); out = lbx, lby, rtx, rty */
+int cu_plane_segment_from_center(double center_x, double center_y, double zoom, int canvas_width,
+                                 int canvas_height, double out[4]);
+/* RenderingKernel.isSegmentBoundsAt{Float,Double}Limit + CudaFractalRenderer.updateFloatPrecision
+ * (cudarenderer/RenderingKernel.java:124-138, cudarenderer/CudaFractalRenderer.java:409-419) */
+int cu_precision_for_segment(double lbx, double lby, double rtx, double rty, int width, int height,
+                             int *precision);
+/* Grid of ⌈w/32⌉×⌈h/32⌉ blocks; CU_ERR_RENDERER beyond 65535 per dimension
+ * (cudarenderer/CudaFractalRenderer.java:242-253). */
+int cu_launch_geometry(int width, int height, unsigned *grid_x, unsigned *grid_y);
+/* ImageHelpers.createDefaultColorPalette (util/ImageHelpers.java:77-111): 1536 entries. */
+int cu_default_palette(uint32_t *out, size_t count);
+/* ImageHelpers.fromBGRAtoRGBA (util/ImageHelpers.java:118-130) */
+int cu_palette_from_argb(const uint32_t *argb, uint32_t *rgba, size_t count);
+
+/* ------------------------------------------------------------------------- */
+/* 3. escape-family layer                                                    */
+/* ------------------------------------------------------------------------- */
+
+enum { CU_FRACTAL_MANDELBROT = 0, CU_FRACTAL_JULIA = 1 };
+enum { CU_SAMPLE_CENTRE = 0, CU_SAMPLE_JITTER = 1, CU_SAMPLE_CORNER = 2 };
+
+/* One rectangular tile of samples.  Sample s of pixel (px, py) is taken at
+ *   X = left + (px + dx) * pixel_w,   Y = top - (py + dy) * pixel_h
+ * (the reference's op order, fractalRendererGeneric.cu:97,119-124) in FP64, or
+ * with the *_f fields in FP32.  (dx, dy) is (0.5, 0.5) for CU_SAMPLE_CENTRE,
+ * (0, 0) for CU_SAMPLE_CORNER, and the seeded SplitMix64 jitter for
+ * CU_SAMPLE_JITTER.  Mandelbrot: z0 = 0, c = (X, Y); Julia: z0 = (X, Y),
+ * c = (julia_cx, julia_cy).  Outputs per pixel: count = the fractal's
+ * computeFractal value (Mandelbrot interior → 0, Julia interior → max_iter),
+ * integer mean over samples; smooth = mean over samples of
+ * n + 1 - log2(ln|z_n|^2 / ln bailout_sq) (0 for an interior sample). */
+typedef struct cu_tile_desc {
+    double left, top, pixel_w, pixel_h;         /* FP64 mapping */
+    float left_f, top_f, pixel_w_f, pixel_h_f;  /* FP32 mapping (float-cast segment) */
+    double julia_cx, julia_cy;
+    double bailout_sq;
+    uint64_t seed;
+    uint64_t out_offset;  /* element offset of pixel (0,0) in the output planes */
+    uint64_t work_offset; /* prefix sum of width*height over earlier tiles */
+    uint32_t width, height;
+    uint32_t out_pitch;   /* elements per output row */
+    uint32_t max_iter;
+    uint32_t kind;        /* CU_FRACTAL_* */
+    uint32_t spp;         /* samples per pixel, >= 1 */
+    uint32_t pattern;     /* CU_SAMPLE_* */
+    uint32_t reserved;
+} cu_tile_desc;
+
+/* Fill a tile from a plane segment: pixel sizes are (rt - lb) / size computed in
+ * FP64 and, for the *_f fields, in FP32 from the float-cast segment (the
+ * reference's KernelMainFloat cast, cudarenderer/KernelMainFloat.java:19-21). */
+int cu_tile_init(cu_tile_desc *tile, double lbx, double lby, double rtx, double rty,
+                 uint32_t width, uint32_t height, uint32_t max_iter, double bailout_sq,
+                 uint32_t kind, double julia_cx, double julia_cy, uint32_t spp, uint32_t pattern,
+                 uint64_t seed);
+
+/* Tuning of the persistent launch (chosen by the autotuner). */
+typedef struct cu_tune {
+    uint32_t block_steps;   /* K: unchecked iterations per block (8, 16, 32)       */
+    uint32_t waves_per_cu;  /* resident waves per CU in the persistent grid         */
+    uint32_t chunk;         /* pixels taken per global-queue grab                    */
+    uint32_t refill_lanes;  /* retire/refill when at least this many lanes are idle  */
+} cu_tune;
+
+typedef struct cu_engine_s *cu_engine;
+/* Loads <kernels_dir>/escape_family.hsaco. */
+int cu_engine_create(const char *kernels_dir, cu_engine *out);
+int cu_engine_destroy(cu_engine e);
+
+/* Variant of the FP64/FP32 step: 0 = exact form ((2x)*y + cy, 7 VALU ops),
+ * 1 = fused form (fma(x*y, 2, cy), 6 VALU ops; equal to the exact form unless
+ * |x*y| falls below the smallest normal), -1 = choose (fused unless the tile's
+ * inputs can reach that range; see cu_tile_fused_safe). */
+int cu_tile_fused_safe(const cu_tile_desc *tile, int precision, int *safe);
+
+/* Straight grid launch over one tile (thread per pixel).  precision: 32 or 64.
+ * d_counts / d_smooth are device planes addressed through out_offset/out_pitch
+ * (d_smooth may be NULL).  *d_iters (device u64, may be NULL) is incremented by
+ * the iterations executed (an interior sample counts max_iter). */
+int cu_escape_tile(cu_engine e, int precision, int variant, const cu_tile_desc *tile,
+                   uint32_t *d_counts, double *d_smooth, unsigned long long *d_iters,
+                   cu_stream stream);
+
+/* Persistent lane-refill launch over n_tiles (<= 1024) tiles whose descriptors
+ * are already in device memory (d_tiles, work_offset = prefix sum of pixel
+ * counts, total_pixels = their sum).  d_queue is a device u64 work counter that
+ * the call clears with a kernel.  variant must be 0 or 1 here.  tune may be
+ * NULL (K=16, 32 waves/CU, chunk 256, refill at 16 lanes). */
+int cu_escape_tiles_persistent(cu_engine e, int precision, int variant,
+                               const cu_tile_desc *d_tiles, uint32_t n_tiles,
+                               uint64_t total_pixels, uint32_t *d_counts, double *d_smooth,
+                               unsigned long long *d_iters, unsigned long long *d_queue,
+                               const cu_tune *tune, cu_stream stream);
+
+/* Palette compose of a smooth plane: idx = L - (round(v) mod L) - 1
+ * (mandelbrot.cu:27-35), RGBA8 out.  The palette is staged in LDS. */
+int cu_compose_smooth(cu_engine e, const double *d_smooth, uint32_t smooth_pitch,
+                      uint32_t *d_rgba, uint32_t rgba_pitch, uint32_t width, uint32_t height,
+                      const uint32_t *d_palette, uint32_t palette_length, cu_stream stream);
+
+/* Elapsed milliseconds of the last engine launch on `stream`, measured with HIP
+ * events recorded on that stream around the kernel. */
+int cu_engine_last_kernel_ms(cu_engine e, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHAOSULTRA_H */

@@ -1,0 +1,139 @@
+"""ctypes view of oracle/liboracle.so — the CPU checker used by the tests.
+
+TEST INFRASTRUCTURE ONLY (also used by __graft_entry__.smoke and bench.py's
+cpu_baseline leg).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+
+_D, _F, _U32, _U64, _I, _P = C.c_double, C.c_float, C.c_uint32, C.c_uint64, C.c_int, C.c_void_p
+_SIG = {
+    "or_mandelbrot_d": (_F, [_U32, _D, _D]),
+    "or_mandelbrot_f": (_F, [_U32, _F, _F]),
+    "or_julia_d": (_F, [_U32, _D, _D, _D, _D]),
+    "or_julia_f": (_F, [_U32, _F, _F, _D, _D]),
+    "or_escape_d": (_U32, [_U32, _D, _D, _D, _D, _D, C.POINTER(_D), C.POINTER(_I)]),
+    "or_escape_f": (_U32, [_U32, _F, _F, _F, _F, _F, C.POINTER(_F), C.POINTER(_I)]),
+    "or_smooth": (_D, [_U32, _D, _D]),
+    "or_splitmix64": (_U64, [_U64]),
+    "or_jitter": (None, [_U64, _U64, C.POINTER(_D), C.POINTER(_D)]),
+    "or_tile_point_d": (None, [_P, _U32, _U32, _U32, C.POINTER(_D), C.POINTER(_D)]),
+    "or_tile_point_f": (None, [_P, _U32, _U32, _U32, C.POINTER(_F), C.POINTER(_F)]),
+    "or_escape_tile_rows": (_U64, [_P, _I, _U32, _U32, _P, _P, _I]),
+    "or_render_main": (None, [_I, _I, _D, _D, _P, C.c_int64, _U32, _U32, _D, _D, _D, _D, _U32, _F,
+                              _U32]),
+    "or_render_advanced": (None, [_I, _I, _D, _D, _P, C.c_int64, _U32, _U32, _D, _D, _D, _D, _U32,
+                                  _F, _U32, _D, _D, _D, _D, _P, C.c_int64, _U32, _U32]),
+    "or_foveation": (None, [_U32, _U32, _U32, _U32, _F, C.POINTER(_F), C.POINTER(_I)]),
+    "or_compose": (None, [_P, C.c_int64, _P, _U32, _U32, _U32, _P, _U32, _F, _I]),
+    "or_compose_smooth": (None, [_P, _U32, _P, _U32, _U32, _U32, _P, _U32]),
+    "or_render_undersampled": (None, [_P, C.c_int64, _U32, _U32, _F, _F, _F, _F, _U32, _U32]),
+    "or_default_palette": (None, [_P]),
+    "or_argb_to_rgba": (_U32, [_U32]),
+}
+
+PIXEL_INFO_DTYPE = np.dtype([("value", "<f4"), ("weight", "<f4"), ("is_reused", "u1"),
+                             ("pad", "u1", (3,)), ("weight_of_new_samples", "<f4")])
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise ImportError(f"{ORACLE_LIB} missing: run `make -C oracle`")
+        l = C.CDLL(ORACLE_LIB)
+        for n, (r, a) in _SIG.items():
+            f = getattr(l, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = l
+    return _lib
+
+
+def ref_lib():
+    """The reference's own computeFractal compiled for the host (this container only)."""
+    if not os.path.exists(REF_LIB):
+        return None
+    l = C.CDLL(REF_LIB)
+    l.ref_mandelbrot_d.restype = _F
+    l.ref_mandelbrot_d.argtypes = [_U32, _D, _D]
+    l.ref_mandelbrot_f.restype = _F
+    l.ref_mandelbrot_f.argtypes = [_U32, _F, _F]
+    l.ref_julia_d.restype = _F
+    l.ref_julia_d.argtypes = [_U32, _D, _D, _D, _D]
+    l.ref_julia_f.restype = _F
+    l.ref_julia_f.argtypes = [_U32, _F, _F, _D, _D]
+    return l
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def escape_tile(tile, precision: int, rows=None, threads: int = 0):
+    """Counts/smooth planes of a tile (tile.out_offset must be 0, pitch = width).
+    Returns (counts uint32[h,w], smooth float64[h,w], iterations executed)."""
+    h, w = tile.height, tile.width
+    r0, r1 = (0, h) if rows is None else rows
+    counts = np.zeros((h, tile.out_pitch), dtype=np.uint32)
+    smooth = np.zeros((h, tile.out_pitch), dtype=np.float64)
+    it = lib().or_escape_tile_rows(C.byref(tile), precision, r0, r1, _ptr(counts), _ptr(smooth),
+                                   threads)
+    return counts[:, :w], smooth[:, :w], int(it)
+
+
+def render_main(precision, kind, julia_c, w, h, seg, max_iter, max_ss, flags):
+    out = np.zeros((h, w), dtype=PIXEL_INFO_DTYPE)
+    lib().or_render_main(precision, kind, julia_c[0], julia_c[1], _ptr(out), w * 16, w, h, *seg,
+                         max_iter, max_ss, flags)
+    return out
+
+
+def render_advanced(precision, kind, julia_c, w, h, seg, max_iter, max_ss, flags, old_seg,
+                    prev: np.ndarray, focus):
+    out = np.zeros((h, w), dtype=PIXEL_INFO_DTYPE)
+    prev = np.ascontiguousarray(prev)
+    lib().or_render_advanced(precision, kind, julia_c[0], julia_c[1], _ptr(out), w * 16, w, h, *seg,
+                             max_iter, max_ss, flags, *old_seg, _ptr(prev), w * 16, focus[0], focus[1])
+    return out
+
+
+def compose(pix: np.ndarray, palette: np.ndarray, max_ss: float, visualise: bool):
+    h, w = pix.shape
+    pix = np.ascontiguousarray(pix)
+    pal = np.ascontiguousarray(palette, dtype=np.uint32)
+    out = np.zeros((h, w), dtype=np.uint32)
+    lib().or_compose(_ptr(pix), w * 16, _ptr(out), w, w, h, _ptr(pal), pal.size, max_ss,
+                     1 if visualise else 0)
+    return out
+
+
+def compose_smooth(smooth: np.ndarray, palette: np.ndarray):
+    h, w = smooth.shape
+    smooth = np.ascontiguousarray(smooth, dtype=np.float64)
+    pal = np.ascontiguousarray(palette, dtype=np.uint32)
+    out = np.zeros((h, w), dtype=np.uint32)
+    lib().or_compose_smooth(_ptr(smooth), w, _ptr(out), w, w, h, _ptr(pal), pal.size)
+    return out
+
+
+def render_undersampled(w, h, seg_f, max_iter, level):
+    out = np.zeros((h, w), dtype=PIXEL_INFO_DTYPE)
+    lib().or_render_undersampled(_ptr(out), w * 16, w, h, *seg_f, max_iter, level)
+    return out
+
+
+def default_palette() -> np.ndarray:
+    out = np.zeros(1536, dtype=np.uint32)
+    lib().or_default_palette(_ptr(out))
+    return out
