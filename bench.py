@@ -62,17 +62,27 @@ class Frame:
         self.counts = cu.DeviceBuffer(n * 4)
         self.smooth = cu.DeviceBuffer(n * 8)
         self.rgba = cu.DeviceBuffer(n * 4)
-        self.iters = cu.DeviceBuffer(8)
+        self.iters = cu.IterationCounter()
         self.queue = cu.DeviceBuffer(8)
         pal = (np.arange(256, dtype=np.uint32) * 0x00010305) | 0xFF000000
         self.palette = cu.DeviceBuffer.from_array(pal.astype(np.uint32))
         self.d_tile = cu.DeviceBuffer(128)
         cu.abi.call("cu_memcpy_htod", self.d_tile.ptr, C.byref(tile), C.c_size_t(128))
         self.variant = 1 if cu.tile_fused_safe(tile, precision) else 0
-        self.render_ms = []
+        self.pipeline = cu.Frame(engine, [tile], precision, -1)
+        self.frame_tune = cu.FrameTune(16, 16, 3, 0)
+        self.render_ms = []   # dominant kernel per step (HIP events on its stream)
+        self.total_ms = []    # whole render per step
+        self.pipe_iters = 0
 
     def step(self, kernel: str):
         cu, e, t = self.cu, self.engine, self.tile
+        if kernel == "pipeline":
+            self.pipeline.render(self.frame_tune, self.palette, 256, self.rgba)
+            p1, tot = self.pipeline.last_ms()
+            self.render_ms.append(p1)
+            self.total_ms.append(tot)
+            return
         if kernel == "persistent":
             e.escape_tiles_persistent(self.precision, self.variant, self.d_tile, 1,
                                       t.width * t.height, self.counts, self.smooth, self.iters,
@@ -80,24 +90,28 @@ class Frame:
         else:
             e.escape_tile(self.precision, t, self.counts, self.smooth, self.iters, self.variant)
         self.render_ms.append(e.last_kernel_ms())
+        self.total_ms.append(self.render_ms[-1])
         e.compose_smooth(self.smooth, t.width, self.rgba, t.width, t.width, t.height, self.palette, 256)
 
-    def iterations(self) -> int:
-        return int(self.iters.download(np.uint64)[0])
+    def iterations(self, kernel: str, steps: int) -> int:
+        if kernel == "pipeline":  # the pipeline clears its counter per render
+            return self.pipeline.iterations() * steps
+        return self.iters.total()
 
 
 def timed(cu, frame: Frame, kernel: str, steps: int, warmup: int):
     for _ in range(warmup):
         frame.step(kernel)
     cu.synchronize()
-    frame.iters.fill_u32(0)
+    frame.iters.clear()
     frame.render_ms = []
+    frame.total_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
         frame.step(kernel)
     cu.synchronize()
     dt = time.perf_counter() - t0
-    return dt, frame.iterations()
+    return dt, frame.iterations(kernel, steps)
 
 
 def cpu_baseline(cu, tile, seconds_target=15.0):
@@ -132,9 +146,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--kernel", choices=["persistent", "grid"], default="persistent")
+    ap.add_argument("--kernel", choices=["pipeline", "persistent", "grid"], default="pipeline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP32 / grid variants")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after timing, copy this many MiB device-to-device once (a plain copy "
+                         "kernel of known size that scales the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -159,11 +176,22 @@ def main():
 
     variants = {}
     if not args.no_variants:
-        other = "grid" if args.kernel == "persistent" else "persistent"
-        dt2, it2 = timed(cu, f64, other, max(2, args.steps // 2), 1)
-        variants[f"fp64_{other}"] = {"giga_iter_s": round(it2 / dt2 / 1e9, 2),
-                                     "render_ms": round(float(np.mean(f64.render_ms)), 3)}
-        log(f"[bench] FP64 {other}: {variants[f'fp64_{other}']}")
+        for other in ("pipeline", "grid", "persistent"):
+            if other == args.kernel:
+                continue
+            dt2, it2 = timed(cu, f64, other, max(2, args.steps // 2), 1)
+            variants[f"fp64_{other}"] = {"giga_iter_s": round(it2 / dt2 / 1e9, 2),
+                                         "render_ms": round(float(np.mean(f64.render_ms)), 3)}
+            log(f"[bench] FP64 {other}: {variants[f'fp64_{other}']}")
+        for ft in ((8, 16, 3), (16, 8, 3), (16, 24, 3), (16, 16, 5)):
+            f64.frame_tune = cu.FrameTune(*ft, 0)
+            dt4, it4 = timed(cu, f64, "pipeline", max(2, args.steps // 2), 1)
+            key = "fp64_pipeline_K%d_h%d_t%d" % ft
+            variants[key] = {"giga_iter_s": round(it4 / dt4 / 1e9, 2),
+                             "pass1_ms": round(float(np.mean(f64.render_ms)), 3),
+                             "render_ms": round(float(np.mean(f64.total_ms)), 3)}
+            log(f"[bench] {key}: {variants[key]}")
+        f64.frame_tune = cu.FrameTune(16, 16, 3, 0)
         f32 = Frame(cu, engine, tile, 32, tune)
         dt3, it3 = timed(cu, f32, args.kernel, max(2, args.steps // 2), 1)
         variants["fp32_plain"] = {"giga_iter_s": round(it3 / dt3 / 1e9, 2),
@@ -204,6 +232,13 @@ def main():
         log("[bench] CPU baseline (oracle, bounded sample)")
         result["cpu_baseline"] = cpu_baseline(cu, tile)
         log(f"[bench] CPU {result['cpu_baseline']['value']} G-iter/s")
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib << 20
+        src, dst = cu.DeviceBuffer(n), cu.DeviceBuffer(n)
+        src.fill_u32(1)
+        cu.abi.call("cu_memcpy_dtod", dst.ptr, src.ptr, n)
+        cu.synchronize()
+        log(f"[bench] copied {args.calibrate_copy_mib} MiB device-to-device for counter calibration")
     engine.close()
     print(json.dumps(result), flush=True)
 

@@ -15,7 +15,7 @@ import numpy as np
 from . import abi
 from .abi import (  # noqa: F401  (re-exported)
     KERNELS_DIR, LIB_PATH, ChaosError, CudaInitializationError, DeviceError, FractalRendererError,
-    IllegalArgumentError, IllegalStateError, RenderingModel, TileDesc, Tune,
+    IllegalArgumentError, IllegalStateError, RenderingModel, TileDesc, Tune, FrameTune,
     FLAG_ADAPTIVE_SS, FLAG_FOVEATION, FLAG_SAMPLE_REUSE, FLAG_VISUALISE_SAMPLE_COUNT,
     FLAG_ZOOMING, FLAG_ZOOMING_IN, FRACTAL_JULIA, FRACTAL_MANDELBROT, PRECISION_DOUBLE,
     PRECISION_SINGLE, PRECISION_TOO_BIG, SAMPLE_CENTRE, SAMPLE_CORNER, SAMPLE_JITTER,
@@ -91,6 +91,22 @@ class DeviceBuffer:
             self.close()
         except Exception:
             pass
+
+
+class IterationCounter(DeviceBuffer):
+    """CU_ITER_SLOTS device u64 slots receiving the iterations executed by a launch."""
+
+    def __init__(self):
+        super().__init__(abi.ITER_SLOTS * 8)
+        self.clear()
+
+    def clear(self) -> None:
+        self.fill_u32(0)
+
+    def total(self) -> int:
+        t = C.c_ulonglong(0)
+        abi.call("cu_iters_total", self.ptr, C.byref(t))
+        return int(t.value)
 
 
 class Stream:
@@ -305,6 +321,9 @@ class Engine:
         abi.call("cu_compose_smooth", self.handle, smooth.ptr, smooth_pitch, rgba.ptr, rgba_pitch,
                  width, height, palette.ptr, palette_length, stream.handle if stream else None)
 
+    def configure(self, tune: Tune) -> None:
+        abi.call("cu_engine_configure", self.handle, C.byref(tune))
+
     def last_kernel_ms(self) -> float:
         ms = C.c_float(0)
         abi.call("cu_engine_last_kernel_ms", self.handle, C.byref(ms))
@@ -314,3 +333,57 @@ class Engine:
         if self.handle and self.handle.value:
             abi.call("cu_engine_destroy", self.handle)
             self.handle = C.c_void_p()
+
+
+class Frame:
+    """The straggler-compaction frame pipeline over a set of tiles (cu_frame)."""
+
+    def __init__(self, engine: Engine, tiles, precision: int = 64, variant: int = -1):
+        tiles = list(tiles)
+        arr = (TileDesc * len(tiles))(*tiles)
+        self.handle = C.c_void_p()
+        abi.call("cu_frame_create", engine.handle, arr, len(tiles), precision, variant,
+                 C.byref(self.handle))
+        counts, smooth, total = C.c_void_p(), C.c_void_p(), C.c_uint64()
+        abi.call("cu_frame_outputs", self.handle, C.byref(counts), C.byref(smooth), C.byref(total))
+        self.d_counts, self.d_smooth, self.total_pixels = counts.value, smooth.value, total.value
+
+    def render(self, tune: FrameTune | None = None, palette: DeviceBuffer | None = None,
+               palette_length: int = 0, rgba: DeviceBuffer | None = None,
+               stream: Stream | None = None) -> None:
+        abi.call("cu_frame_render", self.handle, C.byref(tune) if tune else None,
+                 palette.ptr if palette else None, palette_length, rgba.ptr if rgba else None,
+                 stream.handle if stream else None)
+
+    def counts(self) -> np.ndarray:
+        out = np.empty(self.total_pixels, dtype=np.uint32)
+        abi.call("cu_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), C.c_void_p(self.d_counts),
+                 C.c_size_t(out.nbytes))
+        return out
+
+    def smooth(self) -> np.ndarray:
+        out = np.empty(self.total_pixels, dtype=np.float64)
+        abi.call("cu_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), C.c_void_p(self.d_smooth),
+                 C.c_size_t(out.nbytes))
+        return out
+
+    def iterations(self) -> int:
+        t = C.c_ulonglong(0)
+        abi.call("cu_frame_iterations", self.handle, C.byref(t))
+        return int(t.value)
+
+    def last_ms(self):
+        a, b = C.c_float(0), C.c_float(0)
+        abi.call("cu_frame_last_ms", self.handle, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def close(self) -> None:
+        if self.handle and self.handle.value:
+            abi.call("cu_frame_destroy", self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

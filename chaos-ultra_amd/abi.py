@@ -65,6 +65,7 @@ FLAG_ZOOMING = 1 << 4
 FLAG_ZOOMING_IN = 1 << 5
 FRACTAL_MANDELBROT, FRACTAL_JULIA = 0, 1
 SAMPLE_CENTRE, SAMPLE_JITTER, SAMPLE_CORNER = 0, 1, 2
+ITER_SLOTS = 1024  # CU_ITER_SLOTS
 
 
 class RenderingModel(C.Structure):
@@ -121,6 +122,12 @@ assert C.sizeof(TileDesc) == 128
 class Tune(C.Structure):
     _fields_ = [("block_steps", C.c_uint32), ("waves_per_cu", C.c_uint32),
                 ("chunk", C.c_uint32), ("refill_lanes", C.c_uint32)]
+
+
+class FrameTune(C.Structure):
+    """cu_frame_tune"""
+    _fields_ = [("block_steps", C.c_uint32), ("handoff_lanes", C.c_uint32),
+                ("tail_passes", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 # name -> (restype, argtypes)
@@ -189,6 +196,14 @@ SIGNATURES = {
                                         C.POINTER(Tune), _P]),
     "cu_compose_smooth": (_I, [_P, _P, _U32, _P, _U32, _U32, _U32, _P, _U32, _P]),
     "cu_engine_last_kernel_ms": (_I, [_P, C.POINTER(C.c_float)]),
+    "cu_iters_total": (_I, [_P, C.POINTER(C.c_ulonglong)]),
+    "cu_frame_create": (_I, [_P, C.POINTER(TileDesc), _U32, _I, _I, C.POINTER(_P)]),
+    "cu_frame_destroy": (_I, [_P]),
+    "cu_frame_render": (_I, [_P, C.POINTER(FrameTune), _P, _U32, _P, _P]),
+    "cu_frame_outputs": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_U64)]),
+    "cu_frame_iterations": (_I, [_P, C.POINTER(C.c_ulonglong)]),
+    "cu_frame_last_ms": (_I, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "cu_engine_configure": (_I, [_P, C.POINTER(Tune)]),
 }
 
 _lib = None

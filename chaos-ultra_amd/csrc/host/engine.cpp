@@ -7,19 +7,10 @@
 #include <string>
 
 #include "common.hpp"
+#include "engine_helpers.hpp"
 #include "handles.hpp"
 
-struct cu_engine_s {
-    cu_module module = nullptr;
-    hipEvent_t start = nullptr, stop = nullptr;
-    bool timed = false;
-    std::mutex mutex;
-};
-
 namespace cu {
-namespace {
-
-constexpr uint32_t kMaxPersistentTiles = 1024;  // tile table staged in LDS
 
 cu_function function(cu_engine e, const std::string &name) {
     cu_function f = nullptr;
@@ -82,6 +73,16 @@ void validate_tile(const cu_tile_desc &t) {
     if (!(t.bailout_sq > 0)) fail(CU_ERR_ILLEGAL_ARGUMENT, "bailout_sq must be positive");
 }
 
+void validate_tune(const cu_tune &tn) {
+    if (tn.block_steps != 8 && tn.block_steps != 16 && tn.block_steps != 32)
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "block_steps must be 8, 16 or 32");
+    if (tn.waves_per_cu < 4 || tn.waves_per_cu > 32 || tn.waves_per_cu % 4)
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "waves_per_cu must be a multiple of 4 in [4, 32]");
+    if (tn.chunk == 0 || tn.chunk > (1u << 20)) fail(CU_ERR_ILLEGAL_ARGUMENT, "chunk must be in [1, 2^20]");
+    if (tn.refill_lanes == 0 || tn.refill_lanes > 64)
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "refill_lanes must be in [1, 64]");
+}
+
 bool pick_fused(int variant, const cu_tile_desc &t, int precision) {
     if (variant == 0) return false;
     if (variant == 1) return true;
@@ -89,7 +90,6 @@ bool pick_fused(int variant, const cu_tile_desc &t, int precision) {
     fail(CU_ERR_ILLEGAL_ARGUMENT, "variant must be -1, 0 or 1");
 }
 
-}  // namespace
 }  // namespace cu
 
 using namespace cu;
@@ -171,13 +171,25 @@ int cu_escape_tile(cu_engine e, int precision, int variant, const cu_tile_desc *
         validate_tile(*tile);
         ensure_device();
         const bool fused = pick_fused(variant, *tile, precision);
-        std::string name = std::string("escapeTile") + real_name(precision) + (fused ? "Fused" : "Exact");
-        cu_function f = function(e, name);
         cu_tile_desc t = *tile;
-        void *params[] = {&t, &d_counts, &d_smooth, &d_iters};
-        dim3 block(16, 16, 1);
-        dim3 grid((t.width + 15) / 16, (t.height + 15) / 16, 1);
-        launch(e, f, grid, block, params, stream, true);
+        double smooth_c = smooth_constant(t.bailout_sq);
+        void *params[] = {&t, &smooth_c, &d_counts, &d_smooth, &d_iters};
+        const bool lane_per_sample = t.spp <= 64 && (t.spp & (t.spp - 1)) == 0;
+        std::string name = std::string(lane_per_sample ? "escapeTileSamples" : "escapeTile") +
+                           real_name(precision) + (fused ? "Fused" : "Exact") +
+                           (lane_per_sample && e->tune.block_steps == 8 ? "K8" : "");
+        cu_function f = function(e, name);
+        if (lane_per_sample) {
+            // a wave covers bw x bh pixels (bw*bh = 64/spp), a block 2x2 waves
+            uint32_t ppw = 64 / t.spp, lg = 0;
+            while ((1u << (lg + 1)) <= ppw) ++lg;
+            uint32_t bw = 1u << ((lg + 1) >> 1), bh = ppw / bw;
+            dim3 grid((t.width + 2 * bw - 1) / (2 * bw), (t.height + 2 * bh - 1) / (2 * bh), 1);
+            launch(e, f, grid, dim3(256, 1, 1), params, stream, true);
+        } else {
+            dim3 grid((t.width + 15) / 16, (t.height + 15) / 16, 1);
+            launch(e, f, grid, dim3(16, 16, 1), params, stream, true);
+        }
     });
 }
 
@@ -195,15 +207,11 @@ int cu_escape_tiles_persistent(cu_engine e, int precision, int variant, const cu
                  "the persistent launch takes an explicit variant (check each tile with cu_tile_fused_safe)");
         std::lock_guard<std::mutex> lock(e->mutex);
         ensure_device();
-        cu_tune tn{16, 32, 256, 16};
-        if (tune) tn = *tune;
-        if (tn.block_steps != 8 && tn.block_steps != 16 && tn.block_steps != 32)
-            fail(CU_ERR_ILLEGAL_ARGUMENT, "block_steps must be 8, 16 or 32");
-        if (tn.waves_per_cu < 4 || tn.waves_per_cu > 32 || tn.waves_per_cu % 4)
-            fail(CU_ERR_ILLEGAL_ARGUMENT, "waves_per_cu must be a multiple of 4 in [4, 32]");
-        if (tn.chunk == 0 || tn.chunk > (1u << 20)) fail(CU_ERR_ILLEGAL_ARGUMENT, "chunk must be in [1, 2^20]");
-        if (tn.refill_lanes == 0 || tn.refill_lanes > 64)
-            fail(CU_ERR_ILLEGAL_ARGUMENT, "refill_lanes must be in [1, 64]");
+        cu_tune tn = e->tune;
+        if (tune) {
+            validate_tune(*tune);
+            tn = *tune;
+        }
         if (total == 0) return;
         std::string name = std::string("escapeTilesPersistent") + real_name(precision) +
                            (variant == 1 ? "Fused" : "Exact") + "K" + std::to_string(tn.block_steps);
@@ -241,6 +249,27 @@ int cu_compose_smooth(cu_engine e, const double *d_smooth, uint32_t smooth_pitch
         dim3 block(64, 4, 1);
         dim3 grid((w + 63) / 64, std::min<uint32_t>((h + 3) / 4, 4096), 1);
         launch(e, f, grid, block, params, stream, true);
+    });
+}
+
+int cu_engine_configure(cu_engine e, const cu_tune *tune) {
+    return guarded([&] {
+        if (!e || !tune) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        validate_tune(*tune);
+        std::lock_guard<std::mutex> lock(e->mutex);
+        e->tune = *tune;
+    });
+}
+
+int cu_iters_total(const unsigned long long *d_iters, unsigned long long *total) {
+    return guarded([&] {
+        if (!d_iters || !total) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        ensure_device();
+        unsigned long long slots[CU_ITER_SLOTS];
+        check(hipMemcpy(slots, d_iters, sizeof slots, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        unsigned long long s = 0;
+        for (unsigned long long v : slots) s += v;
+        *total = s;
     });
 }
 

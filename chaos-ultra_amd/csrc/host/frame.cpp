@@ -1,0 +1,302 @@
+// frame.cpp — the straggler-compaction frame pipeline (pass 1, tail passes,
+// combine) over the escape-family module.  See escape_pipeline.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "engine_helpers.hpp"
+#include "handles.hpp"
+
+namespace {
+
+// Device-side argument block of escapePass1 / escapePassQ (same layout as the
+// kernel's PipeArgs).
+struct PipeArgs {
+    const cu_tile_desc *tiles;
+    const uint32_t *block_offset;
+    const uint64_t *sample_offset;
+    const double *smooth_c;
+    uint32_t *sample_count;
+    double *sample_smooth;
+    unsigned long long *iters;
+    uint32_t *counts;
+    double *smooth;
+    uint32_t *rgba;
+    const uint32_t *palette;
+    unsigned char *pending;
+    uint32_t n_tiles;
+    uint32_t handoff;
+    uint32_t subq_capacity;
+    uint32_t palette_length;
+};
+static_assert(sizeof(PipeArgs) == 112, "PipeArgs layout");
+
+constexpr uint32_t kMaxHandoff = 32;   // records a wave may push
+constexpr uint32_t kSubQueues = 1024;  // sub-queues per straggler queue (kernel constant)
+
+void wave_pixel_block(uint32_t spp, uint32_t &bw, uint32_t &bh) {
+    uint32_t ppw = 64 / spp, lg = 0;
+    while ((1u << (lg + 1)) <= ppw) ++lg;
+    bw = 1u << ((lg + 1) >> 1);
+    bh = ppw / bw;
+}
+
+}  // namespace
+
+struct cu_frame_s {
+    cu_engine engine = nullptr;
+    int precision = 64;
+    bool fused = false;
+    std::vector<cu_tile_desc> tiles;
+    uint64_t total_pixels = 0, total_samples = 0;
+    uint32_t total_blocks = 0;
+    uint32_t subq_cap = 0;  // records per sub-queue
+    void *d_tiles = nullptr, *d_block_offset = nullptr, *d_sample_offset = nullptr;
+    void *d_smooth_c = nullptr;  // f64 per tile: log2(log2(bailout_sq))
+    void *d_sample_count = nullptr, *d_sample_smooth = nullptr;
+    void *d_pending = nullptr;   // u8 per pixel
+    void *d_queue[2] = {nullptr, nullptr};
+    void *d_counters = nullptr;  // u32[2][kSubQueues] sub-queue fill counts
+    void *d_counts = nullptr, *d_smooth = nullptr;
+    void *d_iters = nullptr;     // CU_ITER_SLOTS u64
+    hipEvent_t ev_start = nullptr, ev_pass1 = nullptr, ev_end = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool rendered = false;
+};
+
+namespace cu {
+namespace {
+
+void *dalloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) return nullptr;
+    check(hipMalloc(&p, bytes), "hipMalloc");
+    return p;
+}
+
+void free_frame(cu_frame f) {
+    for (void *p : {f->d_tiles, f->d_block_offset, f->d_sample_offset, f->d_smooth_c,
+                    f->d_sample_count, f->d_sample_smooth, f->d_pending, f->d_queue[0],
+                    f->d_queue[1], f->d_counters, f->d_counts, f->d_smooth, f->d_iters})
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : {f->ev_start, f->ev_pass1, f->ev_end})
+        if (e) (void)hipEventDestroy(e);
+    delete f;
+}
+
+void fill(cu_frame f, cu_stream s, void *p, uint32_t value, uint64_t words) {
+    cu_function fn = function(f->engine, "fillU32");
+    void *params[] = {&p, &value, &words};
+    unsigned blocks = (unsigned)std::min<uint64_t>((words + 255) / 256, 1024);
+    launch(f->engine, fn, dim3(blocks), dim3(256), params, s, false);
+}
+
+}  // namespace
+}  // namespace cu
+
+using namespace cu;
+
+extern "C" {
+
+int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, int precision,
+                    int variant, cu_frame *out) {
+    cu_frame f = nullptr;
+    int rc = guarded([&] {
+        if (!e || !tiles || !out) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        if (n_tiles == 0 || n_tiles > kMaxPersistentTiles)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "n_tiles must be in [1, 1024]");
+        real_name(precision);
+        if (variant < -1 || variant > 1) fail(CU_ERR_ILLEGAL_ARGUMENT, "variant must be -1, 0 or 1");
+        f = new cu_frame_s();
+        f->engine = e;
+        f->precision = precision;
+        f->tiles.assign(tiles, tiles + n_tiles);
+        std::vector<uint32_t> block_offset(n_tiles + 1, 0);
+        std::vector<uint64_t> sample_offset(n_tiles, 0);
+        std::vector<double> smooth_c(n_tiles, 0.0);
+        bool all_safe = true;
+        uint64_t blocks = 0, pixels = 0, samples = 0;
+        for (uint32_t k = 0; k < n_tiles; ++k) {
+            cu_tile_desc &t = f->tiles[k];
+            t.out_pitch = t.width;
+            validate_tile(t);
+            if (t.spp > 64 || (t.spp & (t.spp - 1)))
+                fail(CU_ERR_ILLEGAL_ARGUMENT, "frame tiles need spp a power of two <= 64");
+            if ((uint64_t)t.width * t.height * t.spp >= (1ull << 32))
+                fail(CU_ERR_ILLEGAL_ARGUMENT, "tile holds 2^32 samples or more");
+            uint32_t bw, bh;
+            wave_pixel_block(t.spp, bw, bh);
+            block_offset[k] = (uint32_t)blocks;
+            blocks += (uint64_t)((t.width + bw - 1) / bw) * ((t.height + bh - 1) / bh);
+            sample_offset[k] = samples;
+            t.work_offset = pixels;
+            t.out_offset = pixels;
+            pixels += (uint64_t)t.width * t.height;
+            samples += (uint64_t)t.width * t.height * t.spp;
+            smooth_c[k] = smooth_constant(t.bailout_sq);
+            all_safe = all_safe && fused_safe(t, precision);
+        }
+        if (blocks >= (1ull << 31)) fail(CU_ERR_ILLEGAL_ARGUMENT, "frame too large");
+        block_offset[n_tiles] = (uint32_t)blocks;
+        f->fused = variant == 1 || (variant == -1 && all_safe);
+        f->total_blocks = (uint32_t)blocks;
+        f->total_pixels = pixels;
+        f->total_samples = samples;
+        // pass 1 sends wave w to sub-queue w % kSubQueues; a later pass sends a
+        // sub-queue's stragglers back to the same sub-queue, never more than it read
+        f->subq_cap = (uint32_t)((blocks + kSubQueues - 1) / kSubQueues) * kMaxHandoff;
+        const size_t queue_records = (size_t)kSubQueues * f->subq_cap;
+        const size_t rec = precision == 64 ? 32 : 24;
+        ensure_device();
+        f->d_tiles = dalloc(n_tiles * sizeof(cu_tile_desc));
+        f->d_block_offset = dalloc((n_tiles + 1) * sizeof(uint32_t));
+        f->d_sample_offset = dalloc(n_tiles * sizeof(uint64_t));
+        f->d_smooth_c = dalloc(n_tiles * sizeof(double));
+        f->d_sample_count = dalloc(samples * 4);
+        f->d_sample_smooth = dalloc(samples * 8);
+        f->d_pending = dalloc(pixels);
+        f->d_queue[0] = dalloc(queue_records * rec);
+        f->d_queue[1] = dalloc(queue_records * rec);
+        f->d_counters = dalloc(2 * kSubQueues * sizeof(uint32_t));
+        f->d_counts = dalloc(pixels * 4);
+        f->d_smooth = dalloc(pixels * 8);
+        f->d_iters = dalloc(CU_ITER_SLOTS * 8);
+        check(hipMemcpy(f->d_tiles, f->tiles.data(), n_tiles * sizeof(cu_tile_desc),
+                        hipMemcpyHostToDevice), "hipMemcpy H2D");
+        check(hipMemcpy(f->d_block_offset, block_offset.data(), block_offset.size() * 4,
+                        hipMemcpyHostToDevice), "hipMemcpy H2D");
+        check(hipMemcpy(f->d_sample_offset, sample_offset.data(), sample_offset.size() * 8,
+                        hipMemcpyHostToDevice), "hipMemcpy H2D");
+        check(hipMemcpy(f->d_smooth_c, smooth_c.data(), smooth_c.size() * 8,
+                        hipMemcpyHostToDevice), "hipMemcpy H2D");
+        check(hipEventCreate(&f->ev_start), "hipEventCreate");
+        check(hipEventCreate(&f->ev_pass1), "hipEventCreate");
+        check(hipEventCreate(&f->ev_end), "hipEventCreate");
+        *out = f;
+    });
+    if (rc != CU_OK && f) free_frame(f);
+    return rc;
+}
+
+int cu_frame_destroy(cu_frame f) {
+    return guarded([&] {
+        if (!f) return;
+        ensure_device();
+        free_frame(f);
+    });
+}
+
+int cu_frame_render(cu_frame f, const cu_frame_tune *tune, const uint32_t *d_palette,
+                    uint32_t palette_length, uint32_t *d_rgba, cu_stream stream) {
+    return guarded([&] {
+        if (!f) fail(CU_ERR_ILLEGAL_ARGUMENT, "frame is NULL");
+        cu_frame_tune tn{16, 16, 3, 0};
+        if (tune) tn = *tune;
+        if (tn.block_steps != 8 && tn.block_steps != 16)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "block_steps must be 8 or 16");
+        if (tn.handoff_lanes == 0 || tn.handoff_lanes > kMaxHandoff)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "handoff_lanes must be in [1, 32]");
+        if (tn.tail_passes == 0 || tn.tail_passes > 8)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "tail_passes must be in [1, 8]");
+        if ((d_rgba == nullptr) != (d_palette == nullptr))
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "palette and rgba go together");
+        if (d_palette && palette_length == 0) fail(CU_ERR_ILLEGAL_ARGUMENT, "empty palette");
+        std::lock_guard<std::mutex> lock(f->engine->mutex);
+        ensure_device();
+        hipStream_t st = stream ? stream->stream : nullptr;
+        const std::string suffix = std::string(real_name(f->precision)) +
+                                   (f->fused ? "Fused" : "Exact") + "K" +
+                                   std::to_string(tn.block_steps);
+        cu_function pass1 = function(f->engine, "escapePass1" + suffix);
+        cu_function passq = function(f->engine, "escapePassQ" + suffix);
+        cu_function combine = function(f->engine, "combinePending");
+        uint32_t *counters = (uint32_t *)f->d_counters;
+
+        fill(f, stream, f->d_iters, 0, CU_ITER_SLOTS * 2);
+        fill(f, stream, counters, 0, kSubQueues);
+        check(hipEventRecord(f->ev_start, st), "hipEventRecord");
+
+        PipeArgs a{(const cu_tile_desc *)f->d_tiles,
+                   (const uint32_t *)f->d_block_offset,
+                   (const uint64_t *)f->d_sample_offset,
+                   (const double *)f->d_smooth_c,
+                   (uint32_t *)f->d_sample_count,
+                   (double *)f->d_sample_smooth,
+                   (unsigned long long *)f->d_iters,
+                   (uint32_t *)f->d_counts,
+                   (double *)f->d_smooth,
+                   d_rgba,
+                   d_palette,
+                   (unsigned char *)f->d_pending,
+                   (uint32_t)f->tiles.size(),
+                   tn.handoff_lanes,
+                   f->subq_cap,
+                   d_palette ? palette_length : 0u};
+        {
+            uint32_t total_blocks = f->total_blocks;
+            void *q = f->d_queue[0];
+            uint32_t *c = counters;
+            void *params[] = {&a, &total_blocks, &q, &c};
+            launch(f->engine, pass1, dim3((total_blocks + 3) / 4), dim3(256), params, stream, false);
+        }
+        check(hipEventRecord(f->ev_pass1, st), "hipEventRecord");
+        const uint32_t cus = (uint32_t)std::max(1, compute_units());
+        for (uint32_t p = 1; p <= tn.tail_passes; ++p) {
+            const uint32_t in = (p - 1) & 1u, outq = p & 1u;
+            fill(f, stream, counters + outq * kSubQueues, 0, kSubQueues);
+            PipeArgs ap = a;
+            ap.handoff = p == tn.tail_passes ? 0u : tn.handoff_lanes;
+            const void *qin = f->d_queue[in];
+            const uint32_t *cin = counters + in * kSubQueues;
+            void *qout = f->d_queue[outq];
+            uint32_t *cout = counters + outq * kSubQueues;
+            void *params[] = {&ap, &qin, &cin, &qout, &cout};
+            launch(f->engine, passq, dim3(cus * 8), dim3(256), params, stream, false);
+        }
+        {
+            uint64_t total = f->total_pixels;
+            void *params[] = {&a, &total};
+            unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, (uint64_t)cus * 32);
+            launch(f->engine, combine, dim3(blocks), dim3(256), params, stream, false);
+        }
+        check(hipEventRecord(f->ev_end, st), "hipEventRecord");
+        f->last_stream = st;
+        f->rendered = true;
+    });
+}
+
+int cu_frame_outputs(cu_frame f, uint32_t **d_counts, double **d_smooth, uint64_t *total_pixels) {
+    return guarded([&] {
+        if (!f) fail(CU_ERR_ILLEGAL_ARGUMENT, "frame is NULL");
+        if (d_counts) *d_counts = (uint32_t *)f->d_counts;
+        if (d_smooth) *d_smooth = (double *)f->d_smooth;
+        if (total_pixels) *total_pixels = f->total_pixels;
+    });
+}
+
+int cu_frame_iterations(cu_frame f, unsigned long long *total) {
+    return guarded([&] {
+        if (!f || !total) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        if (!f->rendered) fail(CU_ERR_ILLEGAL_STATE, "frame has not been rendered");
+        ensure_device();
+        check(hipStreamSynchronize(f->last_stream), "hipStreamSynchronize");
+        if (cu_iters_total((const unsigned long long *)f->d_iters, total) != CU_OK)
+            fail(CU_ERR_DEVICE, cu_last_error());
+    });
+}
+
+int cu_frame_last_ms(cu_frame f, float *pass1_ms, float *total_ms) {
+    return guarded([&] {
+        if (!f) fail(CU_ERR_ILLEGAL_ARGUMENT, "frame is NULL");
+        if (!f->rendered) fail(CU_ERR_ILLEGAL_STATE, "frame has not been rendered");
+        ensure_device();
+        check(hipEventSynchronize(f->ev_end), "hipEventSynchronize");
+        if (pass1_ms) check(hipEventElapsedTime(pass1_ms, f->ev_start, f->ev_pass1), "hipEventElapsedTime");
+        if (total_ms) check(hipEventElapsedTime(total_ms, f->ev_start, f->ev_end), "hipEventElapsedTime");
+    });
+}
+
+}  // extern "C"

@@ -19,3 +19,15 @@ struct cu_module_s {
 struct cu_stream_s {
     hipStream_t stream;
 };
+
+#include <mutex>
+
+#include "../../../include/chaosultra.h"
+
+struct cu_engine_s {
+    cu_module module = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool timed = false;
+    cu_tune tune{16, 32, 256, 16};  // defaults of every launch (cu_engine_configure)
+    std::mutex mutex;
+};

@@ -9,9 +9,12 @@
 // and |c| <= R the orbit stays outside (|z'| >= |z|^2 - |c| >= |z|(|z|-1) >= |z|),
 // and for Mandelbrot (z0 = 0) a c with |c| > R leaves at step 1 and stays out,
 // so one test at the end of a block tells whether the orbit left inside it.  A
-// block whose end test fails is replayed step by step from its saved start,
-// which yields the exact escape step and the |z|^2 of the failing test; inf/NaN
-// produced after the escape inside the discarded block fail the test as well.
+// block whose end test fails is replayed step by step, which yields the exact
+// escape step and the |z|^2 of the failing test; inf/NaN produced after the
+// escape inside the discarded block fail the test as well.  The block keeps its
+// start and its midpoint: by the same permanence, a midpoint that is still
+// inside means the escape lies in the second half, so a replay runs at most K/2
+// checked steps.
 #pragma once
 #include "cu_types.hpp"
 
@@ -35,6 +38,21 @@ __device__ __forceinline__ void quad_step(Real &x, Real &y, Real cx, Real cy) {
     x = xn;
 }
 
+// Where the replay of a failed block starts: at its midpoint (iteration
+// i + half) when that state is still inside, else at its start.
+template <class Real>
+__device__ __forceinline__ void replay_start(Real &x, Real &y, uint &i, Real sx, Real sy, Real mx,
+                                             Real my, Real bailout_sq, uint half) {
+    if (mx * mx + my * my < bailout_sq) {
+        x = mx;
+        y = my;
+        i += half;
+    } else {
+        x = sx;
+        y = sy;
+    }
+}
+
 // Result of one orbit.
 template <class Real>
 struct Orbit {
@@ -55,10 +73,12 @@ __device__ __forceinline__ Orbit<Real> escape_orbit(Real x, Real y, Real cx, Rea
         while (max_iter - i >= (uint)K) {
             const Real sx = x, sy = y;
 #pragma unroll
-            for (int k = 0; k < K; ++k) quad_step<Real, kFused>(x, y, cx, cy);
+            for (int k = 0; k < K / 2; ++k) quad_step<Real, kFused>(x, y, cx, cy);
+            const Real mx = x, my = y;
+#pragma unroll
+            for (int k = K / 2; k < K; ++k) quad_step<Real, kFused>(x, y, cx, cy);
             if (!(x * x + y * y < bailout_sq)) {
-                x = sx;
-                y = sy;
+                replay_start(x, y, i, sx, sy, mx, my, bailout_sq, (uint)K / 2);
                 break;
             }
             i += K;
@@ -97,12 +117,20 @@ __device__ __forceinline__ bool blocks_valid(double bailout_sq, bool fixed_c, do
     return cx * cx + cy * cy <= bailout_sq * 0.999;
 }
 
-// n + 1 - log2(ln(mag2) / ln(R^2)) — the smooth (continuous) escape value.
-__device__ __forceinline__ double smooth_value(uint n, double mag2, double bailout_sq) {
-    return (double)n + 1.0 - log2(log(mag2) / log(bailout_sq));
+// n + 1 - log2(ln(mag2) / ln(R^2)) — the smooth (continuous) escape value —
+// evaluated as n + 1 - (log2(log2(mag2)) - C) with the per-tile constant
+// C = log2(log2(R^2)) (smooth_constant): two log2 and no division per sample.
+// The two forms differ only in rounding, by about 1e-15 before n + 1 is added.
+__device__ __forceinline__ double smooth_value(uint n, double mag2, double smooth_c) {
+    return (double)n + 1.0 - (log2(log2(mag2)) - smooth_c);
 }
 
-// SplitMix64 finaliser (Steele, Lea, Flood 2014) and the per-sample jitter.
+__device__ __forceinline__ double smooth_constant(double bailout_sq) {
+    return log2(log2(bailout_sq));
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014): output j of the generator seeded with
+// `seed` is mix(seed + (j+1) * 0x9E3779B97F4A7C15).
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -110,11 +138,12 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+// Jitter of sample j: output j of SplitMix64(seed); dx from its high 32 bits,
+// dy from its low 32 bits, each scaled by 2^-32 into [0, 1).
 __device__ __forceinline__ void jitter(uint64_t seed, uint64_t sample_id, double &dx, double &dy) {
-    const uint64_t h1 = splitmix64(seed ^ splitmix64(sample_id));
-    const uint64_t h2 = splitmix64(h1);
-    dx = (double)(h1 >> 11) * 0x1.0p-53;
-    dy = (double)(h2 >> 11) * 0x1.0p-53;
+    const uint64_t h = splitmix64(seed + sample_id * 0x9E3779B97F4A7C15ull);
+    dx = (double)(uint32_t)(h >> 32) * 0x1.0p-32;
+    dy = (double)(uint32_t)h * 0x1.0p-32;
 }
 
 __device__ __forceinline__ void sample_offset(const cu_tile_desc &t, uint px, uint py, uint s,

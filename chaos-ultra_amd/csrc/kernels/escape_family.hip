@@ -28,9 +28,16 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
     return v;
 }
 
+// The iteration counter is CU_ITER_SLOTS u64 slots; each wave adds its total
+// to one slot chosen by its global wave number, so a launch of a million waves
+// does not serialise on one address.  The caller sums the slots.
 __device__ __forceinline__ void add_iterations(unsigned long long *iters, uint64_t v) {
     v = wave_sum(v);
-    if (iters && cu::lane_id() == 0 && v) atomicAdd(iters, (unsigned long long)v);
+    const uint block = blockIdx.x + blockIdx.y * gridDim.x;
+    const uint wave = block * ((blockDim.x * blockDim.y + 63u) >> 6) +
+                      ((threadIdx.x + threadIdx.y * blockDim.x) >> 6);
+    if (iters && cu::lane_id() == 0 && v)
+        atomicAdd(iters + (wave & (CU_ITER_SLOTS - 1u)), (unsigned long long)v);
 }
 
 // Start sample s of pixel (px, py) of tile t: z0 and c for the tile's kind.
@@ -59,10 +66,10 @@ __device__ __forceinline__ bool tile_blocks_valid(const cu_tile_desc &t) {
 }
 
 // ---------------------------------------------------------------------------
-// thread per pixel over one tile
+// thread per pixel over one tile (any spp; samples in sequence)
 // ---------------------------------------------------------------------------
 template <class Real, int K, bool kFused>
-__device__ void escape_tile(const cu_tile_desc &t, uint *counts, double *smooth,
+__device__ void escape_tile(const cu_tile_desc &t, double smooth_c, uint *counts, double *smooth,
                             unsigned long long *iters) {
     const uint px = blockIdx.x * blockDim.x + threadIdx.x;
     const uint py = blockIdx.y * blockDim.y + threadIdx.y;
@@ -80,11 +87,67 @@ __device__ void escape_tile(const cu_tile_desc &t, uint *counts, double *smooth,
                 cu::escape_orbit<Real, K, kFused>(x, y, cx, cy, t.max_iter, bsq, blocks);
             executed += o.n;
             count_sum += o.escaped ? o.n : (julia ? t.max_iter : 0u);
-            smooth_sum += o.escaped ? cu::smooth_value(o.n, (double)o.mag2, t.bailout_sq) : 0.0;
+            smooth_sum += o.escaped ? cu::smooth_value(o.n, (double)o.mag2, smooth_c) : 0.0;
         }
         const uint64_t o = t.out_offset + (uint64_t)py * t.out_pitch + px;
         counts[o] = count_sum / t.spp;
         if (smooth) smooth[o] = smooth_sum / (double)t.spp;
+    }
+    add_iterations(iters, executed);
+}
+
+// ---------------------------------------------------------------------------
+// one sample per lane over one tile (spp a power of two <= 64)
+// ---------------------------------------------------------------------------
+// A wave covers 64/spp pixels as a near-square block (spp 4: 4x4 pixels x 4
+// samples), so neighbouring lanes iterate neighbouring points and the wave's
+// longest orbit stays close to its mean; per-sample setup and the smooth value
+// run on all lanes at once.  The spp lanes of a pixel sum their counts and
+// smooth values in sample order (lane s==0 adds lanes 1..spp-1 in turn), the
+// same order as a sequential loop.  Block: 256 threads = 4 waves in 2x2.
+__device__ __forceinline__ void wave_pixel_block(uint spp, uint &bw, uint &bh) {
+    const uint ppw = 64u / spp;
+    const uint lg = 31u - __clz(ppw);
+    bw = 1u << ((lg + 1u) >> 1);
+    bh = ppw / bw;
+}
+
+template <class Real, int K, bool kFused>
+__device__ void escape_tile_samples(const cu_tile_desc &t, double smooth_c, uint *counts,
+                                    double *smooth, unsigned long long *iters) {
+    const uint spp = t.spp;
+    const uint lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint bw, bh;
+    wave_pixel_block(spp, bw, bh);
+    const uint pix = lane / spp, s = lane % spp;
+    const uint px = (blockIdx.x * 2u + (wave & 1u)) * bw + pix % bw;
+    const uint py = (blockIdx.y * 2u + (wave >> 1)) * bh + pix / bw;
+    const bool inside = px < t.width && py < t.height;
+    const bool julia = t.kind == CU_FRACTAL_JULIA;
+    uint count = 0;
+    double sm = 0;
+    uint64_t executed = 0;
+    if (inside) {
+        Real x, y, cx, cy;
+        begin_sample<Real>(t, px, py, s, x, y, cx, cy);
+        const cu::Orbit<Real> o = cu::escape_orbit<Real, K, kFused>(
+            x, y, cx, cy, t.max_iter, (Real)t.bailout_sq, tile_blocks_valid(t));
+        executed = o.n;
+        count = o.escaped ? o.n : (julia ? t.max_iter : 0u);
+        sm = o.escaped ? cu::smooth_value(o.n, (double)o.mag2, smooth_c) : 0.0;
+    }
+    uint count_sum = count;
+    double smooth_sum = sm;
+    for (uint j = 1; j < spp; ++j) {  // wave-uniform: every lane takes part in the shuffles
+        const uint cj = __shfl(count, (int)(lane + j), 64);
+        const double sj = __shfl(sm, (int)(lane + j), 64);
+        count_sum += cj;
+        smooth_sum += sj;
+    }
+    if (inside && s == 0) {
+        const uint64_t o = t.out_offset + (uint64_t)py * t.out_pitch + px;
+        counts[o] = count_sum / spp;
+        if (smooth) smooth[o] = smooth_sum / (double)spp;
     }
     add_iterations(iters, executed);
 }
@@ -108,7 +171,11 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
                                         unsigned long long *iters, unsigned long long *queue,
                                         uint chunk, uint refill) {
     __shared__ uint64_t lds_offset[kMaxLdsTiles];
-    for (uint k = threadIdx.x; k < n_tiles; k += blockDim.x) lds_offset[k] = tiles[k].work_offset;
+    __shared__ double lds_smooth_c[kMaxLdsTiles];
+    for (uint k = threadIdx.x; k < n_tiles; k += blockDim.x) {
+        lds_offset[k] = tiles[k].work_offset;
+        lds_smooth_c[k] = cu::smooth_constant(tiles[k].bailout_sq);
+    }
     __syncthreads();
 
     const uint lane = cu::lane_id();
@@ -222,7 +289,7 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
             const cu_tile_desc &t = tiles[tile];
             executed += i;
             count_sum += escaped ? i : (t.kind == CU_FRACTAL_JULIA ? max_iter : 0u);
-            smooth_sum += escaped ? cu::smooth_value(i, (double)mag2, t.bailout_sq) : 0.0;
+            smooth_sum += escaped ? cu::smooth_value(i, (double)mag2, lds_smooth_c[tile]) : 0.0;
             ++s;
             if (s < t.spp) {
                 begin_sample<Real>(t, px, py, s, x, y, cx, cy);
@@ -243,8 +310,19 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
 
 #define CU_TILE_KERNELS(NAME, REAL, FUSED)                                                      \
     extern "C" __global__ __launch_bounds__(256) void escapeTile##NAME(                         \
-        cu_tile_desc tile, uint *counts, double *smooth, unsigned long long *iters) {           \
-        escape_tile<REAL, 16, FUSED>(tile, counts, smooth, iters);                              \
+        cu_tile_desc tile, double smooth_c, uint *counts, double *smooth,                       \
+        unsigned long long *iters) {                                                            \
+        escape_tile<REAL, 16, FUSED>(tile, smooth_c, counts, smooth, iters);                    \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void escapeTileSamples##NAME(                  \
+        cu_tile_desc tile, double smooth_c, uint *counts, double *smooth,                       \
+        unsigned long long *iters) {                                                            \
+        escape_tile_samples<REAL, 16, FUSED>(tile, smooth_c, counts, smooth, iters);            \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void escapeTileSamples##NAME##K8(              \
+        cu_tile_desc tile, double smooth_c, uint *counts, double *smooth,                       \
+        unsigned long long *iters) {                                                            \
+        escape_tile_samples<REAL, 8, FUSED>(tile, smooth_c, counts, smooth, iters);             \
     }                                                                                           \
     extern "C" __global__ __launch_bounds__(256) void escapeTilesPersistent##NAME##K8(          \
         const cu_tile_desc *tiles, uint n_tiles, uint64_t total, uint *counts, double *smooth,  \
@@ -296,3 +374,6 @@ extern "C" __global__ __launch_bounds__(256) void fillU32(uint *p, uint value, u
          k += (uint64_t)gridDim.x * blockDim.x)
         p[k] = value;
 }
+
+// The straggler-compaction frame pipeline shares this module's helpers.
+#include "escape_pipeline.hip"

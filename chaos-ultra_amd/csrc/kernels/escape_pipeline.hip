@@ -1,0 +1,340 @@
+// escape_pipeline.hip — the straggler-compaction frame pipeline (part of escape_family.hsaco).
+//
+// A frame is a set of tiles whose spp is a power of two <= 64.  Every sample
+// gets one lane; a pass-1 wave holds 64/spp whole pixels.  Escape counts of
+// neighbouring samples spread widely (in the 4096^2, width-1e-4 frame the median
+// orbit is ~100 steps and the 99th percentile ~500), so a wave that waited for
+// its slowest lane would leave most lanes idle.  Instead, as soon as no more than
+// `handoff` lanes of a wave are still iterating, those lanes push their orbit
+// state (z, i, tile, sample) to a straggler queue and the wave retires; the
+// dispatcher starts fresh, coherent waves in its place.  Later passes run the
+// queue the same way (64 queued orbits per wave), the last one to completion.
+//
+// Pixel results: a pass-1 pixel none of whose samples was queued is reduced in
+// registers (its spp lanes summed in sample order, the order of a sequential
+// loop) and written at once — count, smooth value and palette colour.  Only a
+// pixel with a queued sample goes through the per-sample planes: its finished
+// samples are stored there, its `pending` flag is set, and combinePending sums
+// the planes of flagged pixels once the last pass is done.
+//
+//   escapePass1<Real><Variant>K<K>   wave-blocks of all tiles -> pixels / stragglers
+//   escapePassQ<Real><Variant>K<K>   straggler queue -> samples / stragglers
+//   combinePending                   flagged pixels: per-sample planes -> pixel planes
+
+namespace {
+
+template <class Real>
+struct Straggler {
+    Real x, y;    // orbit state
+    uint i;       // iterations executed so far
+    uint tile;    // tile index
+    uint sample;  // (py * width + px) * spp + s within the tile
+    uint pad;
+};
+static_assert(sizeof(Straggler<double>) == 32, "FP64 straggler record is 32 bytes");
+static_assert(sizeof(Straggler<float>) == 24, "FP32 straggler record is 24 bytes");
+
+struct PipeArgs {
+    const cu_tile_desc *tiles;
+    const uint *block_offset;       // [n_tiles + 1] prefix of wave-blocks (pass 1)
+    const uint64_t *sample_offset;  // [n_tiles] first sample of each tile in the sample planes
+    const double *smooth_c;         // [n_tiles] log2(log2(bailout_sq))
+    uint *sample_count;             // per-sample planes (used by pixels with queued samples)
+    double *sample_smooth;
+    unsigned long long *iters;      // CU_ITER_SLOTS slots
+    uint *counts;                   // pixel planes (tile out_offset + row * out_pitch + column)
+    double *smooth;
+    uint *rgba;                     // palette colour plane, or null
+    const uint *palette;
+    unsigned char *pending;         // per pixel: 1 when its samples finish in later passes
+    uint n_tiles;
+    uint handoff;                   // hand off when running lanes <= handoff (0: run to completion)
+    uint subq_capacity;             // records per sub-queue
+    uint palette_length;
+};
+static_assert(sizeof(PipeArgs) == 112, "PipeArgs layout");
+
+// The straggler queue is kSubQueues sub-queues, each a region of subq_capacity
+// records with its own fill counter: the wave (or queue chunk) number picks the
+// sub-queue, so appends from a million waves spread over 1024 addresses instead
+// of serialising on one counter.
+constexpr uint kSubQueues = 1024;
+
+// Finish one sample: checked steps from (x, y, i) until escape or max_iter; its
+// count follows the fractal's computeFractal convention.
+template <class Real, bool kFused>
+__device__ __forceinline__ void finish_sample(Real x, Real y, Real cx, Real cy, uint i,
+                                              uint max_iter, Real bsq, double smooth_c, bool julia,
+                                              uint &count, double &smooth, uint64_t &executed) {
+    bool escaped = false;
+    Real mag2 = 0;
+    while (i < max_iter) {
+        const Real x2 = x * x;
+        const Real y2 = y * y;
+        const Real m = x2 + y2;
+        if (!(m < bsq)) {
+            escaped = true;
+            mag2 = m;
+            break;
+        }
+        const Real xn = (x2 - y2) + cx;
+        if constexpr (kFused) {
+            y = __builtin_fma(x * y, (Real)2, cy);
+        } else {
+            y = ((x + x) * y) + cy;
+        }
+        x = xn;
+        ++i;
+    }
+    executed += i;
+    count = escaped ? i : (julia ? max_iter : 0u);
+    smooth = escaped ? cu::smooth_value(i, (double)mag2, smooth_c) : 0.0;
+}
+
+// Orbit of one lane with the hand-off rule; shared by all passes.  Returns true
+// when the lane finished its sample (count and smooth set); a lane still
+// running at hand-off is appended to out_queue instead.
+template <class Real, int K, bool kFused>
+__device__ __forceinline__ bool run_lanes(const PipeArgs &a, bool active, Real x, Real y, Real cx,
+                                          Real cy, uint i, uint max_iter, Real bsq, double smooth_c,
+                                          bool julia, bool blocks, uint tile, uint sample,
+                                          Straggler<Real> *out_queue, uint *out_count, uint &count,
+                                          double &smooth) {
+    // mode: 0 none, 1 running blocks, 2 pending (checked finish)
+    int mode = !active ? 0 : ((blocks && max_iter - i >= (uint)K) ? 1 : 2);
+    bool replay = false;
+    Real sx = x, sy = y, mx = x, my = y;
+    for (;;) {
+        const uint64_t run = cu::ballot(mode == 1);
+        if (run == 0) break;
+        if (a.handoff && (uint)__popcll(run) <= a.handoff) break;
+        if (mode == 1) {
+            sx = x;
+            sy = y;
+#pragma unroll
+            for (int k = 0; k < K / 2; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+            mx = x;
+            my = y;
+#pragma unroll
+            for (int k = K / 2; k < K; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+            if (x * x + y * y < bsq) {
+                i += K;
+                if (max_iter - i < (uint)K) mode = 2;
+            } else {
+                mode = 2;
+                replay = true;
+            }
+        }
+    }
+    uint64_t executed = 0;
+    count = 0;
+    smooth = 0;
+    if (mode == 2) {
+        if (replay) cu::replay_start(x, y, i, sx, sy, mx, my, bsq, (uint)K / 2);
+        finish_sample<Real, kFused>(x, y, cx, cy, i, max_iter, bsq, smooth_c, julia, count, smooth,
+                                    executed);
+    }
+    // lanes still running: append to this wave's sub-queue (one atomic per wave)
+    const bool push = mode == 1;
+    const uint64_t pushers = cu::ballot(push);
+    if (pushers) {
+        uint base = 0;
+        const uint first = (uint)__ffsll((unsigned long long)pushers) - 1u;
+        if (cu::lane_id() == first) base = atomicAdd(out_count, (uint)__popcll(pushers));
+        base = __shfl(base, (int)first, 64);
+        if (push) {
+            const uint rank = (uint)__popcll(pushers & ((1ull << cu::lane_id()) - 1ull));
+            Straggler<Real> s;
+            s.x = x;
+            s.y = y;
+            s.i = i;
+            s.tile = tile;
+            s.sample = sample;
+            s.pad = 0;
+            out_queue[base + rank] = s;
+        }
+    }
+    add_iterations(a.iters, executed);
+    return mode == 2;
+}
+
+// Pixel outputs from the in-order sums of its samples: integer mean count,
+// mean smooth value and the palette colour L - (round(smooth) mod L) - 1.
+__device__ __forceinline__ void write_pixel(const PipeArgs &a, uint spp, uint64_t o, uint csum,
+                                            double ssum) {
+    const double sm = ssum / (double)spp;
+    a.counts[o] = csum / spp;
+    if (a.smooth) a.smooth[o] = sm;
+    if (a.rgba) {
+        const uint vi = (uint)round(sm);
+        a.rgba[o] = a.palette[a.palette_length - (vi % a.palette_length) - 1];
+    }
+}
+
+__device__ __forceinline__ void store_sample(const PipeArgs &a, uint tile, uint sample, uint count,
+                                             double smooth) {
+    const uint64_t so = a.sample_offset[tile] + sample;
+    a.sample_count[so] = count;
+    a.sample_smooth[so] = smooth;
+}
+
+// Pass 1: wave w of the launch takes wave-block w of the frame.
+template <class Real, int K, bool kFused>
+__device__ void pass1(const PipeArgs a, uint total_blocks, Straggler<Real> *out_queue,
+                      uint *out_count) {
+    const uint wave_global = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wave_global >= total_blocks) return;  // whole waves only: no barrier below
+    uint lo = 0, hi = a.n_tiles - 1;          // wave-uniform search of the tile
+    while (lo < hi) {
+        const uint mid = (lo + hi + 1) >> 1;
+        if (a.block_offset[mid] <= wave_global) lo = mid; else hi = mid - 1;
+    }
+    const uint tile = lo;
+    const cu_tile_desc &t = a.tiles[tile];
+    const uint spp = t.spp, lane = cu::lane_id();
+    uint bw, bh;
+    wave_pixel_block(spp, bw, bh);
+    const uint b = wave_global - a.block_offset[tile];
+    const uint nbx = (t.width + bw - 1) / bw;
+    const uint pix = lane / spp, s = lane % spp;
+    const uint px = (b % nbx) * bw + pix % bw;
+    const uint py = (b / nbx) * bh + pix / bw;
+    const bool active = px < t.width && py < t.height;
+    Real x = 0, y = 0, cx = 0, cy = 0;
+    if (active) begin_sample<Real>(t, px, py, s, x, y, cx, cy);
+    const uint sample = (py * t.width + px) * spp + s;
+    const uint q = wave_global % kSubQueues;
+    uint count;
+    double sm;
+    const bool done = run_lanes<Real, K, kFused>(
+        a, active, x, y, cx, cy, 0u, t.max_iter, (Real)t.bailout_sq, a.smooth_c[tile],
+        t.kind == CU_FRACTAL_JULIA, tile_blocks_valid(t), tile, sample,
+        out_queue + (uint64_t)q * a.subq_capacity, out_count + q, count, sm);
+
+    // the spp lanes of a pixel are lanes first .. first + spp - 1
+    const uint64_t queued = cu::ballot(active && !done);
+    const uint first = lane - s;
+    const uint64_t group = spp == 64 ? ~0ull : (((1ull << spp) - 1ull) << first);
+    const bool pending = (queued & group) != 0;
+    uint csum = count;
+    double ssum = sm;
+    for (uint j = 1; j < spp; ++j) {  // wave-uniform: every lane takes part in the shuffles
+        csum += __shfl(count, (int)(lane + j), 64);
+        ssum += __shfl(sm, (int)(lane + j), 64);
+    }
+    if (active) {
+        const uint64_t o = t.out_offset + (uint64_t)py * t.out_pitch + px;
+        if (pending) {
+            if (done) store_sample(a, tile, sample, count, sm);
+        } else if (s == 0) {
+            write_pixel(a, spp, o, csum, ssum);
+        }
+        if (s == 0) a.pending[o] = pending ? 1 : 0;
+    }
+}
+
+// Later passes: a fixed grid strides over the chunks of 64 queued orbits.  Chunk
+// c is chunk c / kSubQueues of sub-queue c % kSubQueues, so the low chunks of
+// every sub-queue come first; a chunk past its sub-queue's fill is skipped.
+// Stragglers of sub-queue q go back to sub-queue q of the output queue, which
+// therefore never holds more than the input sub-queue did.
+template <class Real, int K, bool kFused>
+__device__ void passq_chunk(const PipeArgs &a, const Straggler<Real> *in_queue, uint n, uint chunk,
+                            Straggler<Real> *out_queue, uint *out_count) {
+    const uint idx = chunk * 64u + cu::lane_id();
+    const bool active = idx < n;
+    Real x = 0, y = 0, cx = 0, cy = 0, bsq = 4;
+    uint i = 0, tile = 0, sample = 0, max_iter = 0;
+    double smooth_c = 1;
+    bool julia = false, blocks = false;
+    if (active) {
+        const Straggler<Real> e = in_queue[idx];
+        tile = e.tile;
+        sample = e.sample;
+        i = e.i;
+        const cu_tile_desc &t = a.tiles[tile];
+        const uint pixel = sample / t.spp, s = sample % t.spp;
+        Real zx, zy;
+        begin_sample<Real>(t, pixel % t.width, pixel / t.width, s, zx, zy, cx, cy);
+        x = e.x;
+        y = e.y;
+        max_iter = t.max_iter;
+        bsq = (Real)t.bailout_sq;
+        smooth_c = a.smooth_c[tile];
+        julia = t.kind == CU_FRACTAL_JULIA;
+        blocks = tile_blocks_valid(t);
+    }
+    uint count;
+    double sm;
+    if (run_lanes<Real, K, kFused>(a, active, x, y, cx, cy, i, max_iter, bsq, smooth_c, julia,
+                                   blocks, tile, sample, out_queue, out_count, count, sm))
+        store_sample(a, tile, sample, count, sm);
+}
+
+template <class Real, int K, bool kFused>
+__device__ void passq(const PipeArgs a, const Straggler<Real> *in_queue, const uint *in_count,
+                      Straggler<Real> *out_queue, uint *out_count) {
+    const uint wave_global = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint total_waves = gridDim.x * (blockDim.x >> 6);
+    const uint chunks = kSubQueues * ((a.subq_capacity + 63u) / 64u);
+    for (uint c = wave_global; c < chunks; c += total_waves) {
+        const uint q = c % kSubQueues, k = c / kSubQueues;
+        const uint n = in_count[q];
+        if (k * 64u >= n) continue;  // wave-uniform
+        const uint64_t region = (uint64_t)q * a.subq_capacity;
+        passq_chunk<Real, K, kFused>(a, in_queue + region, n, k, out_queue + region, out_count + q);
+    }
+}
+
+}  // namespace
+
+#define CU_PIPE_KERNELS(NAME, REAL, FUSED, STEPS)                                                 \
+    extern "C" __global__ __launch_bounds__(256) void escapePass1##NAME##K##STEPS(                \
+        PipeArgs a, uint total_blocks, Straggler<REAL> *out_queue, uint *out_count) {             \
+        pass1<REAL, STEPS, FUSED>(a, total_blocks, out_queue, out_count);                         \
+    }                                                                                             \
+    extern "C" __global__ __launch_bounds__(256) void escapePassQ##NAME##K##STEPS(                \
+        PipeArgs a, const Straggler<REAL> *in_queue, const uint *in_count,                        \
+        Straggler<REAL> *out_queue, uint *out_count) {                                            \
+        passq<REAL, STEPS, FUSED>(a, in_queue, in_count, out_queue, out_count);                   \
+    }
+
+CU_PIPE_KERNELS(DoubleExact, double, false, 8)
+CU_PIPE_KERNELS(DoubleExact, double, false, 16)
+CU_PIPE_KERNELS(DoubleFused, double, true, 8)
+CU_PIPE_KERNELS(DoubleFused, double, true, 16)
+CU_PIPE_KERNELS(FloatExact, float, false, 8)
+CU_PIPE_KERNELS(FloatExact, float, false, 16)
+CU_PIPE_KERNELS(FloatFused, float, true, 8)
+CU_PIPE_KERNELS(FloatFused, float, true, 16)
+
+// Pixels whose samples finished in several passes: sum the per-sample planes in
+// sample order and write the pixel outputs.  Thread per frame pixel; pixel p lies
+// in the last tile whose work_offset (pixel prefix) is <= p.
+extern "C" __global__ __launch_bounds__(256) void combinePending(PipeArgs a,
+                                                                 uint64_t total_pixels) {
+    __shared__ uint64_t lds_offset[kMaxLdsTiles];
+    for (uint k = threadIdx.x; k < a.n_tiles; k += blockDim.x)
+        lds_offset[k] = a.tiles[k].work_offset;
+    __syncthreads();
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total_pixels;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        if (!a.pending[p]) continue;
+        uint lo = 0, hi = a.n_tiles - 1;
+        while (lo < hi) {
+            const uint mid = (lo + hi + 1) >> 1;
+            if (lds_offset[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        const cu_tile_desc &t = a.tiles[lo];
+        const uint local = (uint)(p - lds_offset[lo]);
+        const uint px = local % t.width, py = local / t.width;
+        const uint64_t so = a.sample_offset[lo] + (uint64_t)local * t.spp;
+        uint csum = 0;
+        double ssum = 0;
+        for (uint s = 0; s < t.spp; ++s) {
+            csum += a.sample_count[so + s];
+            ssum += a.sample_smooth[so + s];
+        }
+        write_pixel(a, t.spp, t.out_offset + (uint64_t)py * t.out_pitch + px, csum, ssum);
+    }
+}

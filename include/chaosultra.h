@@ -252,6 +252,9 @@ typedef struct cu_tune {
 } cu_tune;
 
 typedef struct cu_engine_s *cu_engine;
+/* Default tuning of the engine's launches (the grid launch uses block_steps 8
+ * or 16; the persistent launch uses all four fields when called with NULL). */
+int cu_engine_configure(cu_engine e, const cu_tune *tune);
 /* Loads <kernels_dir>/escape_family.hsaco. */
 int cu_engine_create(const char *kernels_dir, cu_engine *out);
 int cu_engine_destroy(cu_engine e);
@@ -262,10 +265,18 @@ int cu_engine_destroy(cu_engine e);
  * inputs can reach that range; see cu_tile_fused_safe). */
 int cu_tile_fused_safe(const cu_tile_desc *tile, int precision, int *safe);
 
-/* Straight grid launch over one tile (thread per pixel).  precision: 32 or 64.
- * d_counts / d_smooth are device planes addressed through out_offset/out_pitch
- * (d_smooth may be NULL).  *d_iters (device u64, may be NULL) is incremented by
- * the iterations executed (an interior sample counts max_iter). */
+/* Iterations executed (an interior sample counts max_iter) are accumulated in
+ * CU_ITER_SLOTS device u64 slots: each wave adds to one slot, the total is the
+ * sum of the slots (cu_iters_total). */
+#define CU_ITER_SLOTS 1024u
+int cu_iters_total(const unsigned long long *d_iters, unsigned long long *total);
+
+/* Grid launch over one tile.  precision: 32 or 64.  When spp is a power of two
+ * <= 64 each lane takes one sample and a wave covers 64/spp pixels; otherwise
+ * each lane takes one pixel and its samples in sequence.  d_counts / d_smooth
+ * are device planes addressed through out_offset/out_pitch (d_smooth may be
+ * NULL); d_iters (CU_ITER_SLOTS zeroed u64, may be NULL) receives the
+ * iterations executed. */
 int cu_escape_tile(cu_engine e, int precision, int variant, const cu_tile_desc *tile,
                    uint32_t *d_counts, double *d_smooth, unsigned long long *d_iters,
                    cu_stream stream);
@@ -290,6 +301,36 @@ int cu_compose_smooth(cu_engine e, const double *d_smooth, uint32_t smooth_pitch
 /* Elapsed milliseconds of the last engine launch on `stream`, measured with HIP
  * events recorded on that stream around the kernel. */
 int cu_engine_last_kernel_ms(cu_engine e, float *ms);
+
+/* ---- frame pipeline (straggler compaction) ----
+ * A frame owns its tiles (each spp a power of two <= 64) and all device
+ * workspace.  Rendering runs pass 1 over every sample (one lane each), hands the
+ * orbits still running in a wave off to a queue once no more than
+ * handoff_lanes of its 64 lanes iterate, drains the queue in tail_passes
+ * further passes (the last one to completion), and combines the per-sample
+ * results into the pixel planes (count: integer mean; smooth: mean summed in
+ * sample order), optionally composing RGBA8 in the same pass.  Tiles are laid
+ * out back to back in the pixel planes (out_offset/out_pitch/work_offset of the
+ * given tiles are replaced). */
+typedef struct cu_frame_tune {
+    uint32_t block_steps;   /* 8 or 16 */
+    uint32_t handoff_lanes; /* 1..63 */
+    uint32_t tail_passes;   /* 1..8 */
+    uint32_t reserved;
+} cu_frame_tune;
+typedef struct cu_frame_s *cu_frame;
+int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, int precision,
+                    int variant, cu_frame *out);
+int cu_frame_destroy(cu_frame f);
+/* d_palette / d_rgba may be NULL (no compose).  tune may be NULL (defaults). */
+int cu_frame_render(cu_frame f, const cu_frame_tune *tune, const uint32_t *d_palette,
+                    uint32_t palette_length, uint32_t *d_rgba, cu_stream stream);
+/* Device pointers of the pixel planes (total_pixels elements each). */
+int cu_frame_outputs(cu_frame f, uint32_t **d_counts, double **d_smooth, uint64_t *total_pixels);
+/* Iterations executed by the last render (synchronises the frame's stream). */
+int cu_frame_iterations(cu_frame f, unsigned long long *total);
+/* Milliseconds of the last render's passes (pass 1 .. combine), HIP events. */
+int cu_frame_last_ms(cu_frame f, float *pass1_ms, float *total_ms);
 
 #ifdef __cplusplus
 }

@@ -84,7 +84,7 @@ float compute_fractal(int kind, uint32_t max_iter, Real px, Real py, double jcx,
 }
 
 // ---------------------------------------------------------------------------
-// tiles (BASELINE planar form)
+// planar tiles (extension: no reference counterpart)
 // ---------------------------------------------------------------------------
 void sample_offset(const cu_tile_desc &t, uint32_t px, uint32_t py, uint32_t s, double *dx,
                    double *dy) {
@@ -501,11 +501,12 @@ uint64_t or_splitmix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+// Sample j uses output j of SplitMix64 seeded with `seed` (state seed + j*gamma
+// before the generator's own increment): high 32 bits -> dx, low 32 -> dy.
 void or_jitter(uint64_t seed, uint64_t sample_id, double *dx, double *dy) {
-    uint64_t h1 = or_splitmix64(seed ^ or_splitmix64(sample_id));
-    uint64_t h2 = or_splitmix64(h1);
-    *dx = (double)(h1 >> 11) * 0x1.0p-53;
-    *dy = (double)(h2 >> 11) * 0x1.0p-53;
+    uint64_t h = or_splitmix64(seed + sample_id * 0x9E3779B97F4A7C15ull);
+    *dx = (double)(uint32_t)(h >> 32) * 0x1.0p-32;
+    *dy = (double)(uint32_t)h * 0x1.0p-32;
 }
 
 void or_tile_point_d(const cu_tile_desc *t, uint32_t px, uint32_t py, uint32_t s, double *x,

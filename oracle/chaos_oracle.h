@@ -5,9 +5,9 @@
  * by bench.py's cpu_baseline leg as the checker; never by the product path.
  *
  * Each function restates one reference function op for op (citations are
- * reference-relative file:line) with FP contraction off; the BASELINE-defined
- * extensions (bailout radius, smooth value, pixel-centre / jittered samples)
- * say so.
+ * reference-relative file:line) with FP contraction off; the planar-tile
+ * extensions that have no reference counterpart (bailout radius, smooth value,
+ * pixel-centre / jittered samples) say so.
  */
 #ifndef CHAOS_ORACLE_H
 #define CHAOS_ORACLE_H
@@ -36,16 +36,18 @@ float or_julia_d(uint32_t max_iter, double zx, double zy, double julia_cx, doubl
 float or_julia_f(uint32_t max_iter, float zx, float zy, double julia_cx, double julia_cy);
 
 /* The same loop with the bailout constant 4 generalised to bailout_sq
- * (BASELINE).  Returns the iterations executed; *escaped = 1 when the loop
+ * (planar-tile extension).  Returns the iterations executed; *escaped = 1 when the loop
  * stopped on the bailout test, with *mag2 = that test's x*x+y*y. */
 uint32_t or_escape_d(uint32_t max_iter, double zx, double zy, double cx, double cy, double bailout_sq,
                      double *mag2, int *escaped);
 uint32_t or_escape_f(uint32_t max_iter, float zx, float zy, float cx, float cy, float bailout_sq,
                      float *mag2, int *escaped);
-/* n + 1 - log2(ln(mag2) / ln(bailout_sq))  (BASELINE smooth value) */
+/* n + 1 - log2(ln(mag2) / ln(bailout_sq))  (smooth value, planar-tile extension) */
 double or_smooth(uint32_t n, double mag2, double bailout_sq);
 
-/* SplitMix64 finaliser and the per-sample jitter derived from it (BASELINE). */
+/* SplitMix64 finaliser and the per-sample jitter (planar-tile extension): sample j of a
+ * tile takes output j of SplitMix64(seed); dx = high 32 bits * 2^-32,
+ * dy = low 32 bits * 2^-32. */
 uint64_t or_splitmix64(uint64_t z);
 void or_jitter(uint64_t seed, uint64_t sample_id, double *dx, double *dy);
 /* Sample position of (px, py, s) in a tile, FP64 / FP32 mapping. */

@@ -27,12 +27,11 @@ def run_tile(cu, engine, tile, precision, variant):
     w, h = tile.width, tile.height
     counts = cu.DeviceBuffer(w * h * 4)
     smooth = cu.DeviceBuffer(w * h * 8)
-    iters = cu.DeviceBuffer(8)
-    iters.fill_u32(0)
+    iters = cu.IterationCounter()
     engine.escape_tile(precision, tile, counts, smooth, iters, variant=variant)
     cu.synchronize()
     return (counts.download(np.uint32).reshape(h, w), smooth.download(np.float64).reshape(h, w),
-            int(iters.download(np.uint64)[0]))
+            iters.total())
 
 
 def assert_smooth_close(got, want):
@@ -80,6 +79,21 @@ def test_c2_jitter_smooth_vs_oracle(gpu, engine, oracle, precision, variant):
     assert np.array_equal(counts, want_c)
     assert it == want_it
     assert_smooth_close(smooth, want_s)
+
+
+@pytest.mark.parametrize("spp", [1, 2, 3, 8, 16, 32, 64])
+def test_sample_counts_and_lane_layouts(gpu, engine, oracle, spp):
+    """spp a power of two runs one sample per lane (wave = 64/spp pixels, samples
+    summed across lanes in order); other spp run the samples of a pixel in one lane."""
+    cu = gpu
+    seg = cu.plane_segment_from_center(-0.7453, 0.1127, 0.004, 67, 45)
+    tile = cu.make_tile(*seg, 67, 45, 700, 256.0, spp=spp, pattern=cu.SAMPLE_JITTER, seed=spp)
+    for precision in (64, 32):
+        counts, smooth, it = run_tile(cu, engine, tile, precision, -1)
+        want_c, want_s, want_it = oracle.escape_tile(tile, precision)
+        assert np.array_equal(counts, want_c)
+        assert it == want_it
+        assert_smooth_close(smooth, want_s)
 
 
 def test_edge_budgets_and_radii(gpu, engine, oracle):
@@ -133,13 +147,12 @@ def run_persistent(cu, engine, tiles, total, precision, variant, tune):
     cu.abi.call("cu_memcpy_htod", d_tiles.ptr, C.cast(arr, C.c_void_p), C.c_size_t(len(tiles) * 128))
     counts = cu.DeviceBuffer(total * 4)
     smooth = cu.DeviceBuffer(total * 8)
-    iters = cu.DeviceBuffer(8)
-    iters.fill_u32(0)
+    iters = cu.IterationCounter()
     queue = cu.DeviceBuffer(8)
     engine.escape_tiles_persistent(precision, variant, d_tiles, len(tiles), total, counts, smooth,
                                    iters, queue, tune)
     cu.synchronize()
-    return counts.download(np.uint32), smooth.download(np.float64), int(iters.download(np.uint64)[0])
+    return counts.download(np.uint32), smooth.download(np.float64), iters.total()
 
 
 def oracle_tiles(oracle, tiles, total, precision):

@@ -1,0 +1,99 @@
+"""The straggler-compaction frame pipeline on the GPU against the oracle.
+
+Whatever the hand-off threshold, block length and number of tail passes, every
+sample must come out exactly as the oracle's sequential loop computes it: escape
+counts and iteration totals bit-exact, smooth values within 1e-12 relative.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from test_gpu_escape import C2_CENTER, assert_smooth_close, mixed_tiles, oracle_tiles
+
+pytestmark = pytest.mark.gpu
+
+
+def render(cu, engine, tiles, precision, variant, tune, palette=None):
+    f = cu.Frame(engine, tiles, precision, variant)
+    try:
+        rgba = None
+        if palette is not None:
+            pal = cu.DeviceBuffer.from_array(palette)
+            rgba = cu.DeviceBuffer(f.total_pixels * 4)
+            f.render(tune, pal, palette.size, rgba)
+        else:
+            f.render(tune)
+        cu.synchronize()
+        out = (f.counts(), f.smooth(), f.iterations())
+        if rgba is not None:
+            out = out + (rgba.download(np.uint32),)
+        return out
+    finally:
+        f.close()
+
+
+@pytest.mark.parametrize("tune", [(16, 16, 3), (8, 16, 3), (16, 1, 1), (16, 32, 8), (8, 4, 2)])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_c2_frame_pipeline_vs_oracle(gpu, engine, oracle, tune, precision):
+    cu = gpu
+    seg = cu.plane_segment_from_center(*C2_CENTER, 1e-4 / 8, 512, 384)
+    tile = cu.make_tile(*seg, 512, 384, 5000, 256.0, spp=4, pattern=cu.SAMPLE_JITTER, seed=3)
+    counts, smooth, it = render(cu, engine, [tile], precision, -1, cu.FrameTune(*tune, 0))
+    want_c, want_s, want_it = oracle.escape_tile(tile, precision)
+    assert np.array_equal(counts.reshape(384, 512), want_c)
+    assert it == want_it
+    assert_smooth_close(smooth.reshape(384, 512), want_s)
+
+
+def test_c1_frame_pipeline_matches_reference(gpu, engine):
+    cu = gpu
+    with open(os.path.join(GOLDEN, "reference_iteration.json")) as f:
+        g = json.load(f)["c1_mandelbrot_fp64"]
+    tile = cu.make_tile(-2.0, -1.5, 1.0, 1.5, 1024, 1024, 1000, 4.0)
+    counts, _, _ = render(cu, engine, [tile], 64, 0, cu.FrameTune(16, 16, 3, 0))
+    assert hashlib.sha256(counts.astype("<u4").tobytes()).hexdigest() == g["sha256_counts_u32le"]
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_mixed_tiles_pipeline_vs_oracle(gpu, engine, oracle, precision):
+    cu = gpu
+    rng = np.random.default_rng(11 + precision)
+    tiles, total = mixed_tiles(cu, rng, 40, max_side=120)
+    for t in tiles:  # the pipeline needs spp a power of two
+        t.spp = [1, 2, 4][t.spp % 3]
+    counts, smooth, it = render(cu, engine, tiles, precision, 0, cu.FrameTune(16, 8, 4, 0))
+    want_c, want_s, want_it = oracle_tiles(oracle, tiles, total, precision)
+    assert np.array_equal(counts, want_c)
+    assert it == want_it
+    assert_smooth_close(smooth, want_s)
+
+
+def test_fused_compose(gpu, engine, oracle):
+    cu = gpu
+    seg = cu.plane_segment_from_center(*C2_CENTER, 1e-3, 300, 200)
+    tile = cu.make_tile(*seg, 300, 200, 2000, 256.0, spp=4, pattern=cu.SAMPLE_JITTER, seed=5)
+    pal = ((np.arange(256, dtype=np.uint32) * 0x00010305) | 0xFF000000).astype(np.uint32)
+    counts, smooth, it, rgba = render(cu, engine, [tile], 64, -1, None, palette=pal)
+    want_c, want_s, _ = oracle.escape_tile(tile, 64)
+    assert np.array_equal(counts.reshape(200, 300), want_c)
+    assert np.array_equal(rgba.reshape(200, 300), oracle.compose_smooth(want_s, pal))
+
+
+def test_frame_rejects(gpu, engine):
+    cu = gpu
+    t = cu.make_tile(-2, -1, 1, 1, 16, 16, 10, spp=3)
+    with pytest.raises(cu.IllegalArgumentError, match="power of two"):
+        cu.Frame(engine, [t])
+    t = cu.make_tile(-2, -1, 1, 1, 16, 16, 10)
+    f = cu.Frame(engine, [t])
+    try:
+        with pytest.raises(cu.IllegalArgumentError, match="handoff_lanes"):
+            f.render(cu.FrameTune(16, 40, 2, 0))
+        with pytest.raises(cu.IllegalStateError):
+            f.iterations()
+    finally:
+        f.close()

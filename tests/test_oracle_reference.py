@@ -212,6 +212,17 @@ def test_splitmix64_published_vectors(oracle):
     assert outs == [6457827717110365317, 3203168211198807973, 9817491932198370423]
 
 
+def test_jitter_indexes_the_splitmix_sequence(oracle):
+    # sample j takes output j of SplitMix64(seed): high word -> dx, low word -> dy
+    import ctypes as C
+    L = oracle.lib()
+    dx, dy = C.c_double(), C.c_double()
+    for j, h in enumerate([6457827717110365317, 3203168211198807973, 9817491932198370423]):
+        L.or_jitter(1234567, j, C.byref(dx), C.byref(dy))
+        assert dx.value == (h >> 32) / 2 ** 32
+        assert dy.value == (h & 0xFFFFFFFF) / 2 ** 32
+
+
 def test_jitter_is_uniform_and_seeded(oracle):
     import ctypes as C
     L = oracle.lib()
