@@ -3,17 +3,19 @@
 Workload (one step): the 4096x4096 plain-Mandelbrot frame at centre
 (-0.743643887037151, 0.131825904205330), real-axis width 1e-4, 4 seeded-jitter
 samples per pixel, maxIter 5000, bailout |z|^2 >= 256, rendered in FP64 with the
-reference's rounding (bit-exact escape counts) by the persistent lane-refill
-kernel, then composed to RGBA8 through a 256-entry palette.  Inputs are resident
-on the device before timing.  value = Giga-iterations/s (iterations executed,
-an interior sample counts maxIter) over the timed steps.
+reference's rounding (bit-exact escape counts) by the straggler-compaction frame
+pipeline, with the smooth value and the RGBA8 colour through a 256-entry palette.
+Inputs (tile descriptors, palette) are resident on the device before timing.
+value = Giga-iterations/s over the timed steps (iterations executed; an
+interior sample counts maxIter).
 
-Also reported: the FP32 variant of the same frame, the thread-per-pixel kernel,
-the roofline of the dominant kernel (algorithmic FP64 FLOPs = 8 per iteration
-against the 78.6 TFLOP/s vector peak; its add/mul-only ceiling is half that) and
-the oracle's CPU rate on a bounded row sample of the same frame, on the host's cores.
+Also reported: the roofline of the dominant kernel (pass 1 of the pipeline:
+8 algorithmic FP64 FLOPs per iteration it executes, against the 78.6 TFLOP/s
+FP64 vector peak and the 39.3 TFLOP/s add/mul-only ceiling), the thread-per-
+sample grid kernel and the FP32 pipeline on the same frame, and the oracle's CPU
+rate on a bounded row band of the same frame on the host's cores.
 
-python bench.py [--steps K] [--warmup W] [--gpus 1]
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-variants]
 """
 from __future__ import annotations
 
@@ -21,6 +23,7 @@ import argparse
 import importlib
 import json
 import os
+import re
 import sys
 import time
 
@@ -39,7 +42,8 @@ SEED = 20261202
 FLOPS_PER_ITER = 8
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-HBM_PEAK_GBS = 8000.0
+TUNE = (16, 16, 5)  # block_steps, handoff_lanes, tail_passes
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_fp64_pass1_traffic.md")
 
 
 def log(*a):
@@ -48,70 +52,102 @@ def log(*a):
 
 def frame_tile(cu):
     seg = cu.plane_segment_from_center(CENTER[0], CENTER[1], 1e-4, W, H)
-    return seg, cu.make_tile(*seg, W, H, MAX_ITER, BAILOUT_SQ, spp=SPP, pattern=cu.SAMPLE_JITTER,
-                             seed=SEED)
+    return cu.make_tile(*seg, W, H, MAX_ITER, BAILOUT_SQ, spp=SPP, pattern=cu.SAMPLE_JITTER,
+                        seed=SEED)
 
 
-class Frame:
-    """Device buffers of one C2 frame and its launches."""
+def palette(cu):
+    pal = ((np.arange(256, dtype=np.uint32) * 0x00010305) | 0xFF000000).astype(np.uint32)
+    return cu.DeviceBuffer.from_array(pal)
+
+
+class PipelineRun:
+    """One frame object of the pipeline; each step renders it with compose."""
 
     def __init__(self, cu, engine, tile, precision, tune):
-        import ctypes as C
-        self.cu, self.engine, self.tile, self.precision, self.tune = cu, engine, tile, precision, tune
+        self.cu = cu
+        self.frame = cu.Frame(engine, [tile], precision, -1)
+        self.tune = cu.FrameTune(*tune, 0)
+        self.pal = palette(cu)
+        self.rgba = cu.DeviceBuffer(tile.width * tile.height * 4)
+        self.pass1_ms, self.total_ms = [], []
+
+    def step(self):
+        self.frame.render(self.tune, self.pal, 256, self.rgba)
+        p1, tot = self.frame.last_ms()
+        self.pass1_ms.append(p1)
+        self.total_ms.append(tot)
+
+    def reset(self):
+        self.pass1_ms, self.total_ms = [], []
+
+    def iterations(self):
+        return self.frame.iterations(), self.frame.iterations_pass1()
+
+    def close(self):
+        self.frame.close()
+
+
+class GridRun:
+    """Thread-per-sample grid kernel (escape_tile) plus the separate compose."""
+
+    def __init__(self, cu, engine, tile, precision):
+        self.cu, self.engine, self.tile, self.precision = cu, engine, tile, precision
         n = tile.width * tile.height
         self.counts = cu.DeviceBuffer(n * 4)
         self.smooth = cu.DeviceBuffer(n * 8)
         self.rgba = cu.DeviceBuffer(n * 4)
         self.iters = cu.IterationCounter()
-        self.queue = cu.DeviceBuffer(8)
-        pal = (np.arange(256, dtype=np.uint32) * 0x00010305) | 0xFF000000
-        self.palette = cu.DeviceBuffer.from_array(pal.astype(np.uint32))
-        self.d_tile = cu.DeviceBuffer(128)
-        cu.abi.call("cu_memcpy_htod", self.d_tile.ptr, C.byref(tile), C.c_size_t(128))
+        self.pal = palette(cu)
         self.variant = 1 if cu.tile_fused_safe(tile, precision) else 0
-        self.pipeline = cu.Frame(engine, [tile], precision, -1)
-        self.frame_tune = cu.FrameTune(16, 16, 3, 0)
-        self.render_ms = []   # dominant kernel per step (HIP events on its stream)
-        self.total_ms = []    # whole render per step
-        self.pipe_iters = 0
+        self.pass1_ms, self.total_ms = [], []
 
-    def step(self, kernel: str):
-        cu, e, t = self.cu, self.engine, self.tile
-        if kernel == "pipeline":
-            self.pipeline.render(self.frame_tune, self.palette, 256, self.rgba)
-            p1, tot = self.pipeline.last_ms()
-            self.render_ms.append(p1)
-            self.total_ms.append(tot)
-            return
-        if kernel == "persistent":
-            e.escape_tiles_persistent(self.precision, self.variant, self.d_tile, 1,
-                                      t.width * t.height, self.counts, self.smooth, self.iters,
-                                      self.queue, self.tune)
-        else:
-            e.escape_tile(self.precision, t, self.counts, self.smooth, self.iters, self.variant)
-        self.render_ms.append(e.last_kernel_ms())
-        self.total_ms.append(self.render_ms[-1])
-        e.compose_smooth(self.smooth, t.width, self.rgba, t.width, t.width, t.height, self.palette, 256)
+    def step(self):
+        t = self.tile
+        self.engine.escape_tile(self.precision, t, self.counts, self.smooth, self.iters, self.variant)
+        self.pass1_ms.append(self.engine.last_kernel_ms())
+        self.total_ms.append(self.pass1_ms[-1])
+        self.engine.compose_smooth(self.smooth, t.width, self.rgba, t.width, t.width, t.height,
+                                   self.pal, 256)
 
-    def iterations(self, kernel: str, steps: int) -> int:
-        if kernel == "pipeline":  # the pipeline clears its counter per render
-            return self.pipeline.iterations() * steps
-        return self.iters.total()
+    def reset(self):
+        self.pass1_ms, self.total_ms = [], []
+        self.iters.clear()
+
+    def iterations(self):
+        it = self.iters.total()
+        return it, it
+
+    def close(self):
+        pass
 
 
-def timed(cu, frame: Frame, kernel: str, steps: int, warmup: int):
+def timed(cu, run, steps: int, warmup: int):
+    """Warm up, then time exactly `steps` steps between two device synchronisations.
+    Returns (seconds, iterations per step, pass-1 iterations per step)."""
     for _ in range(warmup):
-        frame.step(kernel)
+        run.step()
     cu.synchronize()
-    frame.iters.clear()
-    frame.render_ms = []
-    frame.total_ms = []
+    run.reset()
     t0 = time.perf_counter()
     for _ in range(steps):
-        frame.step(kernel)
+        run.step()
     cu.synchronize()
     dt = time.perf_counter() - t0
-    return dt, frame.iterations(kernel, steps)
+    total, first = run.iterations()
+    if isinstance(run, GridRun):  # the grid counter accumulates over the timed steps
+        total, first = total // steps, first // steps
+    return dt, total, first
+
+
+def read_traffic():
+    """HBM bytes per pass-1 launch from the committed counter summary, if present."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", f.read())
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def cpu_baseline(cu, tile, seconds_target=15.0):
@@ -124,81 +160,76 @@ def cpu_baseline(cu, tile, seconds_target=15.0):
         threads = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    # probe 4 rows to size the sample, then time a band of rows around the centre
-    r0 = H // 2
+    r0 = H // 2  # probe 4 rows to size the sample, then time a band of rows around the centre
     t0 = time.perf_counter()
     _oracle.escape_tile(tile, 64, rows=(r0, r0 + 4), threads=threads)
     probe = max(time.perf_counter() - t0, 1e-3)
-    rows = int(max(4, min(H, 4 * seconds_target / probe)))
-    rows = min(rows, H)
+    rows = int(max(4, min(H, 4 * seconds_target / 3 / probe)))
     a = max(0, r0 - rows // 2)
-    t0 = time.perf_counter()
-    _, _, it = _oracle.escape_tile(tile, 64, rows=(a, a + rows), threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(it / dt / 1e9, 4), "unit": "Giga-iter/s", "cores": threads,
+    rates, it, dts = [], 0, []
+    for _ in range(3):  # median of 3 runs of the band
+        t0 = time.perf_counter()
+        _, _, it = _oracle.escape_tile(tile, 64, rows=(a, a + rows), threads=threads)
+        dts.append(time.perf_counter() - t0)
+        rates.append(it / dts[-1] / 1e9)
+    return {"value": round(float(np.median(rates)), 4), "unit": "Giga-iter/s", "cores": threads,
             "kind": "port",
             "sample": f"rows {a}..{a + rows - 1} of the 4096x4096 FP64 frame (4 spp, maxIter 5000), "
-                      f"{it} iterations in {dt:.1f} s, g++ -O2 -ffp-contract=off, OpenMP {threads} threads"}
+                      f"{it} iterations per run, median of 3 runs ({', '.join(f'{d:.2f}' for d in dts)} s); "
+                      f"g++ -O2 -ffp-contract=off, OpenMP {threads} threads"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--kernel", choices=["pipeline", "persistent", "grid"], default="pipeline")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-variants", action="store_true", help="skip the FP32 / grid variants")
+    ap.add_argument("--no-variants", action="store_true", help="skip the grid / FP32 variants")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (a plain copy "
-                         "kernel of known size that scales the FETCH_SIZE/WRITE_SIZE counters)")
+                         "of known size that scales the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU; this build has no multi-GPU path (asked for "
+              f"--gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
     cu = importlib.import_module("chaos-ultra_amd")
     cu.init(0)
     engine = cu.Engine()
-    seg, tile = frame_tile(cu)
-    tune = cu.Tune(16, 32, 256, 16)
+    tile = frame_tile(cu)
 
-    log(f"[bench] FP64 frame {W}x{H} spp={SPP} maxIter={MAX_ITER} kernel={args.kernel}")
-    f64 = Frame(cu, engine, tile, 64, tune)
-    dt, iters = timed(cu, f64, args.kernel, args.steps, args.warmup)
-    value = iters / dt / 1e9
+    log(f"[bench] FP64 frame {W}x{H} spp={SPP} maxIter={MAX_ITER} pipeline tune={TUNE}")
+    run = PipelineRun(cu, engine, tile, 64, TUNE)
+    dt, iters, iters1 = timed(cu, run, args.steps, args.warmup)
+    value = iters * args.steps / dt / 1e9
     ms_step = dt / args.steps * 1e3
-    kern_ms = float(np.mean(f64.render_ms))
-    per_launch_iters = iters / args.steps
-    achieved_tflops = per_launch_iters * FLOPS_PER_ITER / (kern_ms * 1e-3) / 1e12
-    log(f"[bench] FP64 {value:.1f} G-iter/s, {ms_step:.2f} ms/step, render kernel {kern_ms:.2f} ms")
+    pass1_ms = float(np.mean(run.pass1_ms))
+    render_ms = np.array(run.total_ms)
+    achieved = iters1 * FLOPS_PER_ITER / (pass1_ms * 1e-3) / 1e12
+    frame_tflops = iters * FLOPS_PER_ITER / (float(np.mean(render_ms)) * 1e-3) / 1e12
+    log(f"[bench] FP64 {value:.1f} G-iter/s, {ms_step:.3f} ms/step, pass 1 {pass1_ms:.3f} ms "
+        f"({iters1 / iters:.3f} of the iterations), render p50 {np.median(render_ms):.3f} ms")
+    run.close()
 
     variants = {}
     if not args.no_variants:
-        for other in ("pipeline", "grid", "persistent"):
-            if other == args.kernel:
-                continue
-            dt2, it2 = timed(cu, f64, other, max(2, args.steps // 2), 1)
-            variants[f"fp64_{other}"] = {"giga_iter_s": round(it2 / dt2 / 1e9, 2),
-                                         "render_ms": round(float(np.mean(f64.render_ms)), 3)}
-            log(f"[bench] FP64 {other}: {variants[f'fp64_{other}']}")
-        for ft in ((8, 16, 3), (16, 8, 3), (16, 24, 3), (16, 16, 5)):
-            f64.frame_tune = cu.FrameTune(*ft, 0)
-            dt4, it4 = timed(cu, f64, "pipeline", max(2, args.steps // 2), 1)
-            key = "fp64_pipeline_K%d_h%d_t%d" % ft
-            variants[key] = {"giga_iter_s": round(it4 / dt4 / 1e9, 2),
-                             "pass1_ms": round(float(np.mean(f64.render_ms)), 3),
-                             "render_ms": round(float(np.mean(f64.total_ms)), 3)}
-            log(f"[bench] {key}: {variants[key]}")
-        f64.frame_tune = cu.FrameTune(16, 16, 3, 0)
-        f32 = Frame(cu, engine, tile, 32, tune)
-        dt3, it3 = timed(cu, f32, args.kernel, max(2, args.steps // 2), 1)
-        variants["fp32_plain"] = {"giga_iter_s": round(it3 / dt3 / 1e9, 2),
-                                  "render_ms": round(float(np.mean(f32.render_ms)), 3),
-                                  "note": "FP32 c-grid at this zoom is coarser than the pixel "
-                                          "(pixel 2.4e-8 < ulp_f32 5.96e-8)"}
-        log(f"[bench] FP32: {variants['fp32_plain']}")
+        vsteps = max(3, args.steps // 2)
+        g = GridRun(cu, engine, tile, 64)
+        dt2, it2, _ = timed(cu, g, vsteps, 1)
+        variants["fp64_grid_kernel"] = {"giga_iter_s": round(it2 * vsteps / dt2 / 1e9, 1),
+                                        "kernel_ms": round(float(np.mean(g.pass1_ms)), 3)}
+        log(f"[bench] FP64 grid kernel: {variants['fp64_grid_kernel']}")
+        r32 = PipelineRun(cu, engine, tile, 32, TUNE)
+        dt3, it3, _ = timed(cu, r32, vsteps, 1)
+        variants["fp32_pipeline"] = {
+            "giga_iter_s": round(it3 * vsteps / dt3 / 1e9, 1),
+            "render_ms_p50": round(float(np.median(r32.total_ms)), 3),
+            "note": "FP32 c-grid at this zoom is coarser than the pixel (pixel 2.4e-8 < "
+                    "ulp_f32(0.74) 5.96e-8): a throughput figure, not an image"}
+        log(f"[bench] FP32 pipeline: {variants['fp32_pipeline']}")
+        r32.close()
 
     result = {
         "metric": METRIC,
@@ -213,19 +244,25 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain Mandelbrot FP64 4096x4096, 4 jittered spp, centre "
-                               "(-0.743643887037151, 0.131825904205330), width 1e-4, maxIter 5000, "
-                               "bailout |z|^2>=256, smooth value + RGBA8 compose (256-entry palette)",
-                   "kernel": args.kernel, "variant": "fused" if f64.variant else "exact",
-                   "tune": {"block_steps": tune.block_steps, "waves_per_cu": tune.waves_per_cu,
-                            "chunk": tune.chunk, "refill_lanes": tune.refill_lanes},
-                   "iterations_per_step": int(per_launch_iters)},
-        "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 2),
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4),
-                     "frac_of_add_mul_ceiling": round(achieved_tflops / (FP64_PEAK_TFLOPS / 2), 4),
-                     "kernel_ms": round(kern_ms, 3), "flops_per_iteration": FLOPS_PER_ITER,
-                     "traffic": None},
+        "config": {"workload": "plain Mandelbrot FP64 4096x4096, 4 jittered spp (SplitMix64 seed "
+                               f"{SEED}), centre (-0.743643887037151, 0.131825904205330), width 1e-4, "
+                               "maxIter 5000, bailout |z|^2>=256, smooth value + RGBA8 compose "
+                               "(256-entry palette)",
+                   "kernel": "frame pipeline (pass 1 + %d tail passes + combine)" % TUNE[2],
+                   "tune": {"block_steps": TUNE[0], "handoff_lanes": TUNE[1],
+                            "tail_passes": TUNE[2]},
+                   "iterations_per_step": int(iters),
+                   "render_ms_p50": round(float(np.median(render_ms)), 3),
+                   "render_ms_p99": round(float(np.percentile(render_ms, 99)), 3)},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": read_traffic(),
+                     "kernel": "escapePass1DoubleFusedK%d" % TUNE[0],
+                     "kernel_ms": round(pass1_ms, 4),
+                     "iterations_per_launch": int(iters1),
+                     "flops_per_iteration": FLOPS_PER_ITER,
+                     "frac_of_add_mul_ceiling": round(achieved / (FP64_PEAK_TFLOPS / 2), 4),
+                     "whole_frame_tflops": round(frame_tflops, 2)},
         "variants": variants,
     }
     if not args.no_cpu:

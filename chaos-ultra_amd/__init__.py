@@ -368,9 +368,16 @@ class Frame:
         return out
 
     def iterations(self) -> int:
+        """Iterations executed by the last render, all passes."""
         t = C.c_ulonglong(0)
-        abi.call("cu_frame_iterations", self.handle, C.byref(t))
+        abi.call("cu_frame_iterations", self.handle, C.byref(t), None)
         return int(t.value)
+
+    def iterations_pass1(self) -> int:
+        """Iterations executed by pass 1 of the last render."""
+        t, p = C.c_ulonglong(0), C.c_ulonglong(0)
+        abi.call("cu_frame_iterations", self.handle, C.byref(t), C.byref(p))
+        return int(p.value)
 
     def last_ms(self):
         a, b = C.c_float(0), C.c_float(0)

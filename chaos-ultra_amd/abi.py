@@ -201,7 +201,7 @@ SIGNATURES = {
     "cu_frame_destroy": (_I, [_P]),
     "cu_frame_render": (_I, [_P, C.POINTER(FrameTune), _P, _U32, _P, _P]),
     "cu_frame_outputs": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_U64)]),
-    "cu_frame_iterations": (_I, [_P, C.POINTER(C.c_ulonglong)]),
+    "cu_frame_iterations": (_I, [_P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "cu_frame_last_ms": (_I, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "cu_engine_configure": (_I, [_P, C.POINTER(Tune)]),
 }

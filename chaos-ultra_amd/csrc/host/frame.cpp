@@ -61,7 +61,7 @@ struct cu_frame_s {
     void *d_queue[2] = {nullptr, nullptr};
     void *d_counters = nullptr;  // u32[2][kSubQueues] sub-queue fill counts
     void *d_counts = nullptr, *d_smooth = nullptr;
-    void *d_iters = nullptr;     // CU_ITER_SLOTS u64
+    void *d_iters = nullptr;     // 2 x CU_ITER_SLOTS u64: pass 1, later passes
     hipEvent_t ev_start = nullptr, ev_pass1 = nullptr, ev_end = nullptr;
     hipStream_t last_stream = nullptr;
     bool rendered = false;
@@ -163,7 +163,7 @@ int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, in
         f->d_counters = dalloc(2 * kSubQueues * sizeof(uint32_t));
         f->d_counts = dalloc(pixels * 4);
         f->d_smooth = dalloc(pixels * 8);
-        f->d_iters = dalloc(CU_ITER_SLOTS * 8);
+        f->d_iters = dalloc(2 * CU_ITER_SLOTS * 8);
         check(hipMemcpy(f->d_tiles, f->tiles.data(), n_tiles * sizeof(cu_tile_desc),
                         hipMemcpyHostToDevice), "hipMemcpy H2D");
         check(hipMemcpy(f->d_block_offset, block_offset.data(), block_offset.size() * 4,
@@ -215,7 +215,7 @@ int cu_frame_render(cu_frame f, const cu_frame_tune *tune, const uint32_t *d_pal
         cu_function combine = function(f->engine, "combinePending");
         uint32_t *counters = (uint32_t *)f->d_counters;
 
-        fill(f, stream, f->d_iters, 0, CU_ITER_SLOTS * 2);
+        fill(f, stream, f->d_iters, 0, 2 * CU_ITER_SLOTS * 2);
         fill(f, stream, counters, 0, kSubQueues);
         check(hipEventRecord(f->ev_start, st), "hipEventRecord");
 
@@ -249,6 +249,7 @@ int cu_frame_render(cu_frame f, const cu_frame_tune *tune, const uint32_t *d_pal
             fill(f, stream, counters + outq * kSubQueues, 0, kSubQueues);
             PipeArgs ap = a;
             ap.handoff = p == tn.tail_passes ? 0u : tn.handoff_lanes;
+            ap.iters = (unsigned long long *)f->d_iters + CU_ITER_SLOTS;
             const void *qin = f->d_queue[in];
             const uint32_t *cin = counters + in * kSubQueues;
             void *qout = f->d_queue[outq];
@@ -277,14 +278,18 @@ int cu_frame_outputs(cu_frame f, uint32_t **d_counts, double **d_smooth, uint64_
     });
 }
 
-int cu_frame_iterations(cu_frame f, unsigned long long *total) {
+int cu_frame_iterations(cu_frame f, unsigned long long *total, unsigned long long *pass1) {
     return guarded([&] {
         if (!f || !total) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
         if (!f->rendered) fail(CU_ERR_ILLEGAL_STATE, "frame has not been rendered");
         ensure_device();
         check(hipStreamSynchronize(f->last_stream), "hipStreamSynchronize");
-        if (cu_iters_total((const unsigned long long *)f->d_iters, total) != CU_OK)
+        const unsigned long long *slots = (const unsigned long long *)f->d_iters;
+        unsigned long long first = 0, rest = 0;
+        if (cu_iters_total(slots, &first) != CU_OK || cu_iters_total(slots + CU_ITER_SLOTS, &rest) != CU_OK)
             fail(CU_ERR_DEVICE, cu_last_error());
+        *total = first + rest;
+        if (pass1) *pass1 = first;
     });
 }
 

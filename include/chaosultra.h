@@ -307,14 +307,15 @@ int cu_engine_last_kernel_ms(cu_engine e, float *ms);
  * workspace.  Rendering runs pass 1 over every sample (one lane each), hands the
  * orbits still running in a wave off to a queue once no more than
  * handoff_lanes of its 64 lanes iterate, drains the queue in tail_passes
- * further passes (the last one to completion), and combines the per-sample
- * results into the pixel planes (count: integer mean; smooth: mean summed in
- * sample order), optionally composing RGBA8 in the same pass.  Tiles are laid
+ * further passes (the last one to completion), and writes the pixel planes
+ * (count: integer mean; smooth: mean summed in sample order) and, when a
+ * palette is given, RGBA8.  Pixels whose samples all finish in pass 1 are
+ * written there; the others are combined after the last pass.  Tiles are laid
  * out back to back in the pixel planes (out_offset/out_pitch/work_offset of the
  * given tiles are replaced). */
 typedef struct cu_frame_tune {
     uint32_t block_steps;   /* 8 or 16 */
-    uint32_t handoff_lanes; /* 1..63 */
+    uint32_t handoff_lanes; /* 1..32 */
     uint32_t tail_passes;   /* 1..8 */
     uint32_t reserved;
 } cu_frame_tune;
@@ -327,8 +328,9 @@ int cu_frame_render(cu_frame f, const cu_frame_tune *tune, const uint32_t *d_pal
                     uint32_t palette_length, uint32_t *d_rgba, cu_stream stream);
 /* Device pointers of the pixel planes (total_pixels elements each). */
 int cu_frame_outputs(cu_frame f, uint32_t **d_counts, double **d_smooth, uint64_t *total_pixels);
-/* Iterations executed by the last render (synchronises the frame's stream). */
-int cu_frame_iterations(cu_frame f, unsigned long long *total);
+/* Iterations executed by the last render, in all passes and in pass 1 alone
+ * (pass1 may be NULL).  Synchronises the frame's stream. */
+int cu_frame_iterations(cu_frame f, unsigned long long *total, unsigned long long *pass1);
 /* Milliseconds of the last render's passes (pass 1 .. combine), HIP events. */
 int cu_frame_last_ms(cu_frame f, float *pass1_ms, float *total_ms);
 
