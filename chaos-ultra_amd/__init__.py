@@ -16,6 +16,7 @@ from . import abi
 from .abi import (  # noqa: F401  (re-exported)
     KERNELS_DIR, LIB_PATH, ChaosError, CudaInitializationError, DeviceError, FractalRendererError,
     IllegalArgumentError, IllegalStateError, RenderingModel, TileDesc, Tune, FrameTune,
+    DeepDesc, DeepReport, DEEP_DIRECT_F64, DEEP_PERTURB_F32, DEEP_PERTURB_F64, DEEP_REBASE,
     FLAG_ADAPTIVE_SS, FLAG_FOVEATION, FLAG_SAMPLE_REUSE, FLAG_VISUALISE_SAMPLE_COUNT,
     FLAG_ZOOMING, FLAG_ZOOMING_IN, FRACTAL_JULIA, FRACTAL_MANDELBROT, PRECISION_DOUBLE,
     PRECISION_SINGLE, PRECISION_TOO_BIG, SAMPLE_CENTRE, SAMPLE_CORNER, SAMPLE_JITTER,
@@ -387,6 +388,77 @@ class Frame:
     def close(self) -> None:
         if self.handle and self.handle.value:
             abi.call("cu_frame_destroy", self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# deep zoom (cu_deep_*)
+# ---------------------------------------------------------------------------
+def make_deep(centre_x, centre_y, width_plane, width, height, max_iter, bailout_sq=256.0,
+              rebase=True, max_references=8, glitch_eps=1e-6, centre_lo=(0.0, 0.0)) -> DeepDesc:
+    """A deep frame of `width` x `height` square pixels spanning `width_plane` of the
+    real axis, centred on (centre_x + centre_lo[0], centre_y + centre_lo[1])."""
+    pw = width_plane / width
+    return DeepDesc(centre_x, centre_y, centre_lo[0], centre_lo[1], pw, pw, bailout_sq, width,
+                    height, max_iter, DEEP_REBASE if rebase else 0, max_references, glitch_eps)
+
+
+class DeepFrame:
+    """One deep-zoom frame (cu_deep): direct FP64, or FP32/FP64 perturbation around a
+    double-double reference orbit computed on the GPU."""
+
+    def __init__(self, engine: Engine, desc: DeepDesc, method: int = DEEP_PERTURB_F64):
+        self.desc, self.method = desc, method
+        self.handle = C.c_void_p()
+        abi.call("cu_deep_create", engine.handle, C.byref(desc), method, C.byref(self.handle))
+        counts, smooth, glitch = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        abi.call("cu_deep_outputs", self.handle, C.byref(counts), C.byref(smooth), C.byref(glitch))
+        self._planes = (counts.value, smooth.value, glitch.value)
+        self.pixels = desc.width * desc.height
+
+    def render(self, palette: DeviceBuffer | None = None, palette_length: int = 0,
+               rgba: DeviceBuffer | None = None, stream: Stream | None = None) -> None:
+        abi.call("cu_deep_render", self.handle, palette.ptr if palette else None, palette_length,
+                 rgba.ptr if rgba else None, stream.handle if stream else None)
+
+    def _download(self, which: int, dtype) -> np.ndarray:
+        out = np.empty(self.pixels, dtype=dtype)
+        abi.call("cu_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), C.c_void_p(self._planes[which]),
+                 C.c_size_t(out.nbytes))
+        return out.reshape(self.desc.height, self.desc.width)
+
+    def counts(self) -> np.ndarray:
+        return self._download(0, np.uint32)
+
+    def smooth(self) -> np.ndarray:
+        return self._download(1, np.float64)
+
+    def glitched(self) -> np.ndarray:
+        return self._download(2, np.uint8)
+
+    def report(self) -> DeepReport:
+        r = DeepReport()
+        abi.call("cu_deep_stats", self.handle, C.byref(r))
+        return r
+
+    def orbit(self) -> np.ndarray:
+        """The primary reference orbit of the last render, shape (M + 1, 4)."""
+        dt = np.float32 if self.method == DEEP_PERTURB_F32 else np.float64
+        buf = np.empty((self.desc.max_iter + 1) * 4, dtype=dt)
+        M = C.c_uint32(0)
+        abi.call("cu_deep_orbit", self.handle, buf.ctypes.data_as(C.c_void_p), self.desc.max_iter + 1,
+                 C.byref(M))
+        return buf[: (M.value + 1) * 4].reshape(-1, 4)
+
+    def close(self) -> None:
+        if self.handle and self.handle.value:
+            abi.call("cu_deep_destroy", self.handle)
             self.handle = C.c_void_p()
 
     def __del__(self):

@@ -130,6 +130,33 @@ class FrameTune(C.Structure):
                 ("tail_passes", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class DeepDesc(C.Structure):
+    """cu_deep_desc (80 bytes)"""
+    _fields_ = [("centre_x", C.c_double), ("centre_y", C.c_double),
+                ("centre_x_lo", C.c_double), ("centre_y_lo", C.c_double),
+                ("pixel_w", C.c_double), ("pixel_h", C.c_double),
+                ("bailout_sq", C.c_double),
+                ("width", C.c_uint32), ("height", C.c_uint32),
+                ("max_iter", C.c_uint32), ("flags", C.c_uint32),
+                ("max_references", C.c_uint32), ("glitch_eps", C.c_float)]
+
+
+assert C.sizeof(DeepDesc) == 80
+
+
+class DeepReport(C.Structure):
+    """cu_deep_report"""
+    _fields_ = [("iterations", C.c_ulonglong), ("reference_length", C.c_uint32),
+                ("references", C.c_uint32), ("glitched", C.c_uint32), ("reserved", C.c_uint32),
+                ("orbit_ms", C.c_float), ("render_ms", C.c_float), ("total_ms", C.c_float),
+                ("reserved2", C.c_float)]
+
+
+DEEP_DIRECT_F64 = 1
+DEEP_PERTURB_F32 = 32
+DEEP_PERTURB_F64 = 64
+DEEP_REBASE = 1
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _I = C.c_int
@@ -204,6 +231,12 @@ SIGNATURES = {
     "cu_frame_iterations": (_I, [_P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "cu_frame_last_ms": (_I, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "cu_engine_configure": (_I, [_P, C.POINTER(Tune)]),
+    "cu_deep_create": (_I, [_P, C.POINTER(DeepDesc), _I, C.POINTER(_P)]),
+    "cu_deep_destroy": (_I, [_P]),
+    "cu_deep_render": (_I, [_P, _P, _U32, _P, _P]),
+    "cu_deep_outputs": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "cu_deep_stats": (_I, [_P, C.POINTER(DeepReport)]),
+    "cu_deep_orbit": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
 }
 
 _lib = None

@@ -1,6 +1,7 @@
 // engine.cpp — layer 3 of the C ABI: launches of the escape-family module
 // (escape_family.hsaco) for the planar tile forms, the persistent launch and
 // the smooth compose, with HIP-event timing on the launch stream.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -81,6 +82,20 @@ void validate_tune(const cu_tune &tn) {
     if (tn.chunk == 0 || tn.chunk > (1u << 20)) fail(CU_ERR_ILLEGAL_ARGUMENT, "chunk must be in [1, 2^20]");
     if (tn.refill_lanes == 0 || tn.refill_lanes > 64)
         fail(CU_ERR_ILLEGAL_ARGUMENT, "refill_lanes must be in [1, 64]");
+}
+
+void *device_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) return nullptr;
+    check(hipMalloc(&p, bytes), "hipMalloc");
+    return p;
+}
+
+void fill_u32(cu_engine e, cu_stream s, void *p, uint32_t value, uint64_t words) {
+    cu_function fn = function(e, "fillU32");
+    void *params[] = {&p, &value, &words};
+    const unsigned blocks = (unsigned)std::min<uint64_t>((words + 255) / 256, 1024);
+    launch(e, fn, dim3(blocks), dim3(256), params, s, false);
 }
 
 bool pick_fused(int variant, const cu_tile_desc &t, int precision) {

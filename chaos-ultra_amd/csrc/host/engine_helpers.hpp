@@ -19,6 +19,9 @@ bool fused_safe(const cu_tile_desc &t, int precision);
 void validate_tile(const cu_tile_desc &t);
 void validate_tune(const cu_tune &tn);
 bool pick_fused(int variant, const cu_tile_desc &t, int precision);
+void *device_alloc(size_t bytes);  // hipMalloc, nullptr for 0 bytes; throws CuError
+// fill `words` u32 at p with `value` by the fillU32 kernel (safe under graph capture)
+void fill_u32(cu_engine e, cu_stream s, void *p, uint32_t value, uint64_t words);
 
 // C = log2(log2(R^2)), the per-tile term of the smooth value
 // n + 1 - (log2(log2(|z|^2)) - C) computed by the kernels.

@@ -70,12 +70,7 @@ struct cu_frame_s {
 namespace cu {
 namespace {
 
-void *dalloc(size_t bytes) {
-    void *p = nullptr;
-    if (bytes == 0) return nullptr;
-    check(hipMalloc(&p, bytes), "hipMalloc");
-    return p;
-}
+void *dalloc(size_t bytes) { return device_alloc(bytes); }
 
 void free_frame(cu_frame f) {
     for (void *p : {f->d_tiles, f->d_block_offset, f->d_sample_offset, f->d_smooth_c,
@@ -88,10 +83,7 @@ void free_frame(cu_frame f) {
 }
 
 void fill(cu_frame f, cu_stream s, void *p, uint32_t value, uint64_t words) {
-    cu_function fn = function(f->engine, "fillU32");
-    void *params[] = {&p, &value, &words};
-    unsigned blocks = (unsigned)std::min<uint64_t>((words + 255) / 256, 1024);
-    launch(f->engine, fn, dim3(blocks), dim3(256), params, s, false);
+    fill_u32(f->engine, s, p, value, words);
 }
 
 }  // namespace

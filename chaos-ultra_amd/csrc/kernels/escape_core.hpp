@@ -22,6 +22,11 @@
 
 namespace cu {
 
+// Fused multiply-add with one rounding in the operands' own precision
+// (__builtin_fma alone is the double builtin: float operands would be promoted).
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // One step.  kFused computes y' as fma(x*y, 2, cy): 2*(x*y) is exact, so this
 // equals ((2x)*y)+cy whenever x*y is zero or at least the smallest normal —
 // one VALU op fewer (6 per step instead of 7).
@@ -31,7 +36,7 @@ __device__ __forceinline__ void quad_step(Real &x, Real &y, Real cx, Real cy) {
     const Real y2 = y * y;
     const Real xn = (x2 - y2) + cx;
     if constexpr (kFused) {
-        y = __builtin_fma(x * y, (Real)2, cy);
+        y = fma_r(x * y, (Real)2, cy);
     } else {
         y = ((x + x) * y) + cy;
     }
@@ -98,7 +103,7 @@ __device__ __forceinline__ Orbit<Real> escape_orbit(Real x, Real y, Real cx, Rea
         }
         const Real xn = (x2 - y2) + cx;
         if constexpr (kFused) {
-            y = __builtin_fma(x * y, (Real)2, cy);
+            y = fma_r(x * y, (Real)2, cy);
         } else {
             y = ((x + x) * y) + cy;
         }

@@ -279,7 +279,7 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
                 }
                 const Real xn = (x2 - y2) + cx;
                 if constexpr (kFused) {
-                    y = __builtin_fma(x * y, (Real)2, cy);
+                    y = cu::fma_r(x * y, (Real)2, cy);
                 } else {
                     y = ((x + x) * y) + cy;
                 }
@@ -377,3 +377,5 @@ extern "C" __global__ __launch_bounds__(256) void fillU32(uint *p, uint value, u
 
 // The straggler-compaction frame pipeline shares this module's helpers.
 #include "escape_pipeline.hip"
+// The deep-zoom perturbation form likewise.
+#include "perturb.hip"

@@ -79,7 +79,7 @@ __device__ __forceinline__ void finish_sample(Real x, Real y, Real cx, Real cy, 
         }
         const Real xn = (x2 - y2) + cx;
         if constexpr (kFused) {
-            y = __builtin_fma(x * y, (Real)2, cy);
+            y = cu::fma_r(x * y, (Real)2, cy);
         } else {
             y = ((x + x) * y) + cy;
         }

@@ -1,0 +1,317 @@
+// perturb.hip — deep-zoom render form (part of escape_family.hsaco).
+//
+// Below a pixel of ~1e-13 of the plane the reference's double kernel cannot
+// tell pixels apart (cudarenderer/RenderingKernel.java:124-138 and
+// cudarenderer/CudaFractalRenderer.java:217-220 stop there).  This form
+// iterates every pixel as a small delta around one reference orbit:
+//
+//   z_n = Z_m + d_n,   d_{n+1} = (2 Z_m + d_n) d_n + dc,   m <- m + 1
+//
+// The reference orbit Z is computed once per frame on the GPU by one lane in
+// double-double (FP64 error-free transformations, ~106-bit significand): an
+// orbit computed in plain FP64 drifts from the true orbit long before the
+// pixel scale of a 1e-20 frame.  It is stored rounded to the delta type,
+// each entry {Zx, Zy, |Z|^2/2, 0}.  Deltas are FP32 or FP64 (same VALU rate on
+// gfx950 without packed FP32; FP64 deltas are far more precise).
+//
+// Rebasing (Zhuoran): when |z| < |d| — tested as fma(Zx, dx, fma(Zy, dy,
+// |Z|^2/2)) < 0 — or when the pixel outlives the reference (m == M), the pixel
+// restarts the reference from Z_0 = 0 with d = z.  Without rebasing,
+// Pauldelbrot's test |z|^2 < eps |Z|^2 marks a pixel glitched and the host
+// re-renders glitched pixels around a secondary reference.
+//
+// Hot loop: unchecked blocks of K steps (one orbit load and 8 FMA-class
+// VALU ops per step); the rebase test is a per-step ballot OR-ed into a lane
+// mask, escape is tested once at the block end (escape is permanent for
+// bailout_sq >= 4, |c| <= 2), and a block with either event is replayed with
+// checked steps — the same operation sequence as the oracle's checked loop.
+//
+//   referenceOrbit{Float,Double}   one lane: double-double orbit of C -> entries, M
+//   perturb{Float,Double}          lane per pixel (8x8 pixels per wave)
+//   deepDirectDouble               direct FP64 at the same pixel -> c mapping
+
+namespace {
+
+// ---- double-double (Dekker / Knuth error-free transformations) ----
+struct DD {
+    double hi, lo;
+};
+
+__device__ __forceinline__ DD dd_two_sum(double a, double b) {
+    const double s = a + b;
+    const double bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DD dd_quick_two_sum(double a, double b) {
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+    DD s = dd_two_sum(a.hi, b.hi);
+    const DD t = dd_two_sum(a.lo, b.lo);
+    double e = s.lo + t.hi;
+    s = dd_quick_two_sum(s.hi, e);
+    e = s.lo + t.lo;
+    return dd_quick_two_sum(s.hi, e);
+}
+__device__ __forceinline__ DD dd_mul(DD a, DD b) {
+    const double p = a.hi * b.hi;
+    const double pe = __builtin_fma(a.hi, b.hi, -p);  // double operands
+    const double cross = a.hi * b.lo + a.lo * b.hi;
+    return dd_quick_two_sum(p, pe + cross);
+}
+
+template <class Real>
+struct OrbitEntry {
+    Real x, y, h, pad;  // Z_m and |Z_m|^2 / 2
+};
+
+template <class Real>
+__device__ void reference_orbit(double cx_hi, double cx_lo, double cy_hi, double cy_lo,
+                                uint max_iter, double bailout_sq, OrbitEntry<Real> *orbit,
+                                uint *length) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const DD cx{cx_hi, cx_lo}, cy{cy_hi, cy_lo};
+    DD x{0.0, 0.0}, y{0.0, 0.0};
+    uint n = 0;
+    for (;;) {
+        const double x2 = x.hi * x.hi;
+        const double y2 = y.hi * y.hi;
+        const double mag = x2 + y2;
+        OrbitEntry<Real> e;
+        e.x = (Real)x.hi;
+        e.y = (Real)y.hi;
+        e.h = (Real)(0.5 * mag);
+        e.pad = 0;
+        orbit[n] = e;
+        if (!(mag < bailout_sq) || n == max_iter) break;
+        const DD xx = dd_mul(x, x);
+        const DD yy = dd_mul(y, y);
+        const DD xy = dd_mul(x, y);
+        const DD xn = dd_add(dd_add(xx, DD{-yy.hi, -yy.lo}), cx);
+        y = dd_add(DD{xy.hi + xy.hi, xy.lo + xy.lo}, cy);
+        x = xn;
+        ++n;
+    }
+    *length = n;
+}
+
+struct DeepArgs {
+    const void *orbit;         // OrbitEntry<Real>[M + 1]
+    const uint *orbit_len;     // M
+    uint *counts;              // W x H, row-major
+    double *smooth;
+    unsigned char *glitch;     // 1 = glitched (Pauldelbrot) in the last pass over the pixel
+    uint *rgba;                // optional palette colour
+    const uint *palette;
+    unsigned long long *iters; // CU_ITER_SLOTS slots
+    double centre_x, centre_y; // direct form only: c = centre + offset
+    double pixel_w, pixel_h;
+    double ref_ox, ref_oy;     // reference point - frame centre (plane units)
+    double bailout_sq;
+    double smooth_c;           // log2(log2(bailout_sq))
+    uint width, height, max_iter, flags;
+    uint palette_length;
+    float glitch_eps;
+    uint only_glitched;        // 1: pixels whose glitch flag is set, others untouched
+    uint pad;
+};
+static_assert(sizeof(DeepArgs) == 160, "DeepArgs layout");
+
+constexpr uint kDeepRebase = 1u;
+
+__device__ __forceinline__ void deep_offset(const DeepArgs &a, uint px, uint py, double &ox,
+                                            double &oy) {
+    ox = ((double)px + 0.5 - (double)a.width * 0.5) * a.pixel_w;
+    oy = ((double)a.height * 0.5 - ((double)py + 0.5)) * a.pixel_h;
+}
+
+// One checked step of the perturbation loop (the oracle's or_perturb_pixel
+// body).  Returns 0 to continue, 1 escaped (mag set), 2 max_iter, 3 glitched.
+template <class Real>
+__device__ __forceinline__ int checked_step(const OrbitEntry<Real> *Z, uint M, Real &dx, Real &dy,
+                                            uint &m, uint &n, Real dcx, Real dcy, uint max_iter,
+                                            Real bsq, bool rebase, Real geps, Real &mag_out) {
+    const OrbitEntry<Real> e = Z[m];
+    Real Zx = e.x, Zy = e.y;
+    const Real zx = Zx + dx;
+    const Real zy = Zy + dy;
+    const Real mag = cu::fma_r(zx, zx, zy * zy);
+    if (!(n < max_iter)) return 2;
+    if (!(mag < bsq)) {
+        mag_out = mag;
+        return 1;
+    }
+    if (rebase) {
+        const Real t = cu::fma_r(Zx, dx, cu::fma_r(Zy, dy, e.h));
+        if (t < (Real)0 || m == M) {
+            dx = zx;
+            dy = zy;
+            m = 0;
+            Zx = 0;
+            Zy = 0;
+        }
+    } else if (m == M) {
+        dx = zx;
+        dy = zy;
+        m = 0;
+        Zx = 0;
+        Zy = 0;
+    } else if (m > 0 && mag < geps * (e.h + e.h)) {
+        mag_out = mag;
+        return 3;
+    }
+    const Real ax = cu::fma_r((Real)2, Zx, dx);
+    const Real ay = cu::fma_r((Real)2, Zy, dy);
+    const Real ndx = cu::fma_r(ax, dx, cu::fma_r(-ay, dy, dcx));
+    const Real ndy = cu::fma_r(ax, dy, cu::fma_r(ay, dx, dcy));
+    dx = ndx;
+    dy = ndy;
+    ++m;
+    ++n;
+    return 0;
+}
+
+// Whole pixel; lanes whose `active` is false only take part in the ballots.
+template <class Real, int K>
+__device__ int perturb_pixel(const OrbitEntry<Real> *Z, uint M, Real dcx, Real dcy, uint max_iter,
+                             Real bsq, bool rebase, Real geps, bool active, uint &n_out,
+                             Real &mag_out) {
+    Real dx = 0, dy = 0;
+    uint m = 0, n = 0;
+    int state = active ? 0 : 4;
+    mag_out = 0;
+    const uint lane = cu::lane_id();
+    for (;;) {
+        if (cu::ballot(state == 0) == 0) break;
+        const bool blk = state == 0 && rebase && max_iter - n >= (uint)K && M - m >= (uint)K;
+        bool replay = false;
+        if (blk) {
+            const Real sx = dx, sy = dy;
+            const uint sm = m, sn = n;
+            const OrbitEntry<Real> *zp = Z + m;
+            uint64_t flagged = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const OrbitEntry<Real> e = zp[k];
+                const Real t = cu::fma_r(e.x, dx, cu::fma_r(e.y, dy, e.h));
+                flagged |= cu::ballot(t < (Real)0);
+                const Real ax = cu::fma_r((Real)2, e.x, dx);
+                const Real ay = cu::fma_r((Real)2, e.y, dy);
+                const Real ndx = cu::fma_r(ax, dx, cu::fma_r(-ay, dy, dcx));
+                const Real ndy = cu::fma_r(ax, dy, cu::fma_r(ay, dx, dcy));
+                dx = ndx;
+                dy = ndy;
+            }
+            m += K;
+            n += K;
+            const OrbitEntry<Real> e = Z[m];
+            const Real zx = e.x + dx;
+            const Real zy = e.y + dy;
+            const Real mag = cu::fma_r(zx, zx, zy * zy);
+            if (((flagged >> lane) & 1ull) || !(mag < bsq)) {
+                dx = sx;
+                dy = sy;
+                m = sm;
+                n = sn;
+                replay = true;
+            }
+        }
+        if (state == 0 && (!blk || replay)) {
+            const int steps = blk ? K : 1;
+            for (int s = 0; s < steps && state == 0; ++s)
+                state = checked_step<Real>(Z, M, dx, dy, m, n, dcx, dcy, max_iter, bsq, rebase,
+                                           geps, mag_out);
+        }
+    }
+    n_out = n;
+    return state;
+}
+
+__device__ __forceinline__ void deep_write(const DeepArgs &a, uint64_t o, int state, uint n,
+                                           double mag) {
+    const uint count = (state == 1 || state == 3) ? n : 0u;  // Mandelbrot: max_iter -> 0
+    const double sm = state == 1 ? cu::smooth_value(n, mag, a.smooth_c) : 0.0;
+    a.counts[o] = count;
+    if (a.smooth) a.smooth[o] = sm;
+    if (a.glitch) a.glitch[o] = state == 3 ? 1 : 0;
+    if (a.rgba) {
+        const uint vi = (uint)round(sm);
+        a.rgba[o] = a.palette[a.palette_length - (vi % a.palette_length) - 1];
+    }
+}
+
+// 256 threads = 2 x 2 waves of 8 x 8 pixels.
+__device__ __forceinline__ void deep_pixel_coords(uint &px, uint &py) {
+    const uint lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    px = (blockIdx.x * 2u + (wave & 1u)) * 8u + (lane & 7u);
+    py = (blockIdx.y * 2u + (wave >> 1)) * 8u + (lane >> 3);
+}
+
+template <class Real, int K>
+__device__ void perturb_frame(const DeepArgs a) {
+    uint px, py;
+    deep_pixel_coords(px, py);
+    const uint64_t o = (uint64_t)py * a.width + px;
+    bool active = px < a.width && py < a.height;
+    if (active && a.only_glitched) active = a.glitch[o] != 0;
+    Real dcx = 0, dcy = 0;
+    if (active) {
+        double ox, oy;
+        deep_offset(a, px, py, ox, oy);
+        dcx = (Real)(ox - a.ref_ox);
+        dcy = (Real)(oy - a.ref_oy);
+    }
+    const uint M = *a.orbit_len;
+    uint n = 0;
+    Real mag = 0;
+    const int state = perturb_pixel<Real, K>(
+        (const OrbitEntry<Real> *)a.orbit, M, dcx, dcy, a.max_iter, (Real)a.bailout_sq,
+        (a.flags & kDeepRebase) != 0, (Real)a.glitch_eps, active, n, mag);
+    if (active) deep_write(a, o, state, n, (double)mag);
+    add_iterations(a.iters, active ? (uint64_t)n : 0u);
+}
+
+// Direct FP64 at c = centre + offset: the reference loop with bailout_sq.
+template <int K, bool kFused>
+__device__ void deep_direct(const DeepArgs a) {
+    uint px, py;
+    deep_pixel_coords(px, py);
+    const bool active = px < a.width && py < a.height;
+    uint64_t executed = 0;
+    if (active) {
+        double ox, oy;
+        deep_offset(a, px, py, ox, oy);
+        const double cx = a.centre_x + ox, cy = a.centre_y + oy;
+        const cu::Orbit<double> r = cu::escape_orbit<double, K, kFused>(
+            0.0, 0.0, cx, cy, a.max_iter, a.bailout_sq, a.bailout_sq >= 4.0);
+        executed = r.n;
+        deep_write(a, (uint64_t)py * a.width + px, r.escaped ? 1 : 2, r.n, r.mag2);
+    }
+    add_iterations(a.iters, executed);
+}
+
+}  // namespace
+
+extern "C" __global__ __launch_bounds__(64) void referenceOrbitFloat(
+    double cx_hi, double cx_lo, double cy_hi, double cy_lo, uint max_iter, double bailout_sq,
+    OrbitEntry<float> *orbit, uint *length) {
+    reference_orbit<float>(cx_hi, cx_lo, cy_hi, cy_lo, max_iter, bailout_sq, orbit, length);
+}
+extern "C" __global__ __launch_bounds__(64) void referenceOrbitDouble(
+    double cx_hi, double cx_lo, double cy_hi, double cy_lo, uint max_iter, double bailout_sq,
+    OrbitEntry<double> *orbit, uint *length) {
+    reference_orbit<double>(cx_hi, cx_lo, cy_hi, cy_lo, max_iter, bailout_sq, orbit, length);
+}
+extern "C" __global__ __launch_bounds__(256) void perturbFloat(DeepArgs a) {
+    perturb_frame<float, 8>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void perturbDouble(DeepArgs a) {
+    perturb_frame<double, 8>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void deepDirectDoubleExact(DeepArgs a) {
+    deep_direct<16, false>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void deepDirectDoubleFused(DeepArgs a) {
+    deep_direct<16, true>(a);
+}

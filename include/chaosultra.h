@@ -334,6 +334,62 @@ int cu_frame_iterations(cu_frame f, unsigned long long *total, unsigned long lon
 /* Milliseconds of the last render's passes (pass 1 .. combine), HIP events. */
 int cu_frame_last_ms(cu_frame f, float *pass1_ms, float *total_ms);
 
+/* ---- deep zoom: perturbation around a double-double reference orbit ----
+ * No reference counterpart: the reference stops at its double kernel
+ * (cudarenderer/RenderingKernel.java:124-138; cudarenderer/CudaFractalRenderer.java:217-220
+ * only logs below the double ulp).  A W x H Mandelbrot frame centred on
+ * C = (centre_x + centre_x_lo, centre_y + centre_y_lo): pixel (px, py) samples
+ * C + ((px + 0.5 - W/2) * pixel_w, (H/2 - (py + 0.5)) * pixel_h), offsets in
+ * double.  Counts follow mandelbrot.cu:22-23 (max_iter -> 0); bailout |z|^2 >=
+ * bailout_sq.  Methods:
+ *   CU_DEEP_DIRECT_F64   direct FP64 at the double-rounded c (the reference
+ *                        rounding; bit-exact with the oracle)
+ *   CU_DEEP_PERTURB_F32  FP32 deltas around one reference orbit at C computed on
+ *   CU_DEEP_PERTURB_F64  FP64 deltas    the GPU in double-double, stored rounded
+ * flags CU_DEEP_REBASE: Zhuoran rebasing (restart the reference when |z| < |delta|).
+ * Without it, Pauldelbrot's test |z|^2 < glitch_eps |Z|^2 marks glitched pixels and
+ * cu_deep_render re-renders them around up to max_references secondary
+ * reference orbits (the glitched pixel nearest the glitched centroid). */
+#define CU_DEEP_DIRECT_F64 1
+#define CU_DEEP_PERTURB_F32 32
+#define CU_DEEP_PERTURB_F64 64
+#define CU_DEEP_REBASE 1u
+typedef struct cu_deep_desc {
+    double centre_x, centre_y;
+    double centre_x_lo, centre_y_lo;
+    double pixel_w, pixel_h;
+    double bailout_sq;        /* >= 4 */
+    uint32_t width, height;
+    uint32_t max_iter;
+    uint32_t flags;
+    uint32_t max_references;  /* secondary references per render, <= 16 */
+    float glitch_eps;         /* Pauldelbrot threshold on |z|^2 / |Z|^2, e.g. 1e-6 */
+} cu_deep_desc;
+typedef struct cu_deep_report {
+    unsigned long long iterations; /* executed by the last render, all passes */
+    uint32_t reference_length;     /* M: last index of the primary reference orbit */
+    uint32_t references;           /* reference orbits computed by the last render */
+    uint32_t glitched;             /* pixels still glitched after the last render */
+    uint32_t reserved;
+    float orbit_ms;                /* primary reference orbit */
+    float render_ms;               /* first pass over all pixels */
+    float total_ms;                /* whole render, glitch rounds included */
+    float reserved2;
+} cu_deep_report;
+typedef struct cu_deep_s *cu_deep;
+int cu_deep_create(cu_engine e, const cu_deep_desc *d, int method, cu_deep *out);
+int cu_deep_destroy(cu_deep d);
+/* d_palette / d_rgba may be NULL.  Synchronous when glitch rounds run. */
+int cu_deep_render(cu_deep d, const uint32_t *d_palette, uint32_t palette_length,
+                   uint32_t *d_rgba, cu_stream stream);
+/* Device planes, W*H each (row-major): counts, smooth values, glitch flags (u8). */
+int cu_deep_outputs(cu_deep d, uint32_t **d_counts, double **d_smooth, uint8_t **d_glitch);
+/* Statistics of the last render (synchronises). */
+int cu_deep_stats(cu_deep d, cu_deep_report *out);
+/* Copy of the primary reference orbit of the last render: entries of 4 values
+ * {Zx, Zy, |Z|^2/2, 0} in the delta precision; *length = M (entries M + 1). */
+int cu_deep_orbit(cu_deep d, void *host_out, uint32_t capacity_entries, uint32_t *length);
+
 #ifdef __cplusplus
 }
 #endif

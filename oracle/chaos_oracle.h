@@ -101,6 +101,34 @@ void or_render_undersampled(or_pixel_info *out, int64_t out_pitch_bytes, uint32_
 void or_default_palette(uint32_t *out /* 1536 */);
 uint32_t or_argb_to_rgba(uint32_t argb);
 
+/* ---- deep zoom (deep_oracle.cpp): FP32 perturbation, direct FP64 and
+ * double-double oracles; pixel (px, py) of a W x H frame centred on C samples
+ * C + ((px + 0.5 - W/2) * pw, (H/2 - (py + 0.5)) * ph). ---- */
+void or_dd_add(double a_hi, double a_lo, double b_hi, double b_lo, double *hi, double *lo);
+uint32_t or_reference_orbit_f(double cx_hi, double cx_lo, double cy_hi, double cy_lo,
+                              uint32_t max_iter, double bailout_sq, float *orbit);
+uint32_t or_reference_orbit_d(double cx_hi, double cx_lo, double cy_hi, double cy_lo,
+                              uint32_t max_iter, double bailout_sq, double *orbit);
+uint32_t or_perturb_pixel_f(const float *orbit, uint32_t M, float dcx, float dcy,
+                            uint32_t max_iter, float bailout_sq, uint32_t flags, float glitch_eps,
+                            float *mag_out, int *state);
+uint32_t or_perturb_pixel_d(const double *orbit, uint32_t M, double dcx, double dcy,
+                            uint32_t max_iter, double bailout_sq, uint32_t flags,
+                            double glitch_eps, double *mag_out, int *state);
+void or_deep_offset(uint32_t px, uint32_t py, uint32_t width, uint32_t height, double pixel_w,
+                    double pixel_h, double *ox, double *oy);
+uint32_t or_deep_direct_d(double cx, double cy, double ox, double oy, uint32_t max_iter,
+                          double bailout_sq, double *mag_out, int *escaped);
+uint32_t or_deep_direct_dd(double cx_hi, double cx_lo, double cy_hi, double cy_lo, double ox,
+                           double oy, uint32_t max_iter, double bailout_sq, double *mag_out,
+                           int *escaped);
+uint64_t or_deep_pixels(int method, const uint32_t *px, const uint32_t *py, uint64_t n_pixels,
+                        uint32_t width, uint32_t height, double cx_hi, double cx_lo, double cy_hi,
+                        double cy_lo, double pixel_w, double pixel_h, uint32_t max_iter,
+                        double bailout_sq, const void *orbit, uint32_t M, double ref_ox,
+                        double ref_oy, uint32_t flags, float glitch_eps, uint32_t *counts,
+                        double *mag, int32_t *state, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,7 +38,19 @@ _SIG = {
     "or_render_undersampled": (None, [_P, C.c_int64, _U32, _U32, _F, _F, _F, _F, _U32, _U32]),
     "or_default_palette": (None, [_P]),
     "or_argb_to_rgba": (_U32, [_U32]),
+    "or_dd_add": (None, [_D, _D, _D, _D, C.POINTER(_D), C.POINTER(_D)]),
+    "or_reference_orbit_f": (_U32, [_D, _D, _D, _D, _U32, _D, _P]),
+    "or_reference_orbit_d": (_U32, [_D, _D, _D, _D, _U32, _D, _P]),
+    "or_perturb_pixel_f": (_U32, [_P, _U32, _F, _F, _U32, _F, _U32, _F, C.POINTER(_F), C.POINTER(_I)]),
+    "or_perturb_pixel_d": (_U32, [_P, _U32, _D, _D, _U32, _D, _U32, _D, C.POINTER(_D), C.POINTER(_I)]),
+    "or_deep_offset": (None, [_U32, _U32, _U32, _U32, _D, _D, C.POINTER(_D), C.POINTER(_D)]),
+    "or_deep_direct_d": (_U32, [_D, _D, _D, _D, _U32, _D, C.POINTER(_D), C.POINTER(_I)]),
+    "or_deep_direct_dd": (_U32, [_D, _D, _D, _D, _D, _D, _U32, _D, C.POINTER(_D), C.POINTER(_I)]),
+    "or_deep_pixels": (_U64, [_I, _P, _P, _U64, _U32, _U32, _D, _D, _D, _D, _D, _D, _U32, _D, _P,
+                              _U32, _D, _D, _U32, _F, _P, _P, _P, _I]),
 }
+
+DEEP_PERTURB_F32, DEEP_DIRECT_F64, DEEP_DIRECT_DD, DEEP_PERTURB_F64 = 0, 1, 2, 3
 
 PIXEL_INFO_DTYPE = np.dtype([("value", "<f4"), ("weight", "<f4"), ("is_reused", "u1"),
                              ("pad", "u1", (3,)), ("weight_of_new_samples", "<f4")])
@@ -137,3 +149,43 @@ def default_palette() -> np.ndarray:
     out = np.zeros(1536, dtype=np.uint32)
     lib().or_default_palette(_ptr(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# deep zoom
+# ---------------------------------------------------------------------------
+def dd_add(a_hi, a_lo, b_hi, b_lo=0.0):
+    hi, lo = C.c_double(), C.c_double()
+    lib().or_dd_add(a_hi, a_lo, b_hi, b_lo, C.byref(hi), C.byref(lo))
+    return hi.value, lo.value
+
+
+def reference_orbit(cx, cy, max_iter, bailout_sq, precision=64, lo=(0.0, 0.0)):
+    """Double-double reference orbit rounded to the delta type: (entries (M+1, 4), M)."""
+    dt = np.float64 if precision == 64 else np.float32
+    buf = np.zeros((max_iter + 1) * 4, dtype=dt)
+    f = lib().or_reference_orbit_d if precision == 64 else lib().or_reference_orbit_f
+    M = f(cx, lo[0], cy, lo[1], max_iter, bailout_sq, buf.ctypes.data_as(C.c_void_p))
+    return buf[: (M + 1) * 4].reshape(-1, 4).copy(), M
+
+
+def deep_pixels(method, px, py, desc, orbit=None, M=0, ref=(0.0, 0.0), threads=None):
+    """Pixels (px, py) of the deep frame `desc` (a cu DeepDesc-like object) by `method`:
+    returns (counts u32, mag f64, state i32, total iterations)."""
+    px = np.ascontiguousarray(px, dtype=np.uint32)
+    py = np.ascontiguousarray(py, dtype=np.uint32)
+    n = px.size
+    counts = np.zeros(n, np.uint32)
+    mag = np.zeros(n, np.float64)
+    state = np.zeros(n, np.int32)
+    if orbit is None:
+        orbit = np.zeros(4, np.float64)
+    orbit = np.ascontiguousarray(orbit)
+    threads = threads or (os.cpu_count() or 1)
+    total = lib().or_deep_pixels(
+        method, px.ctypes.data_as(C.c_void_p), py.ctypes.data_as(C.c_void_p), n, desc.width,
+        desc.height, desc.centre_x, desc.centre_x_lo, desc.centre_y, desc.centre_y_lo, desc.pixel_w,
+        desc.pixel_h, desc.max_iter, desc.bailout_sq, orbit.ctypes.data_as(C.c_void_p), M, ref[0],
+        ref[1], desc.flags, desc.glitch_eps, counts.ctypes.data_as(C.c_void_p),
+        mag.ctypes.data_as(C.c_void_p), state.ctypes.data_as(C.c_void_p), threads)
+    return counts, mag, state, int(total)

@@ -1,0 +1,168 @@
+"""The deep-zoom oracle (oracle/deep_oracle.cpp) pinned without the GPU.
+
+* double-double addition is exact (checked with rational arithmetic);
+* the double-double reference orbit follows the 300-bit orbit of the same c
+  (mpmath) to within one FP64 rounding at every step, escapes at the same step,
+  and that step differs from the plain FP64 orbit's — the reason the reference
+  orbit is not computed in plain FP64;
+* the double-double direct oracle agrees with 300-bit iteration pixel by pixel;
+* the committed double-double fixture is what the oracle computes;
+* perturbation with a zero delta reproduces the reference orbit's escape step,
+  and at a shallow zoom FP64 perturbation agrees with direct FP64;
+* without rebasing, glitched pixels re-rendered around a secondary reference at
+  the glitched pixel nearest their centroid come out as with rebasing.
+"""
+import importlib
+import math
+import os
+from fractions import Fraction
+
+import mpmath
+import numpy as np
+import pytest
+
+import _oracle as O
+from conftest import GOLDEN
+
+CENTRE = (-0.743643887037151, 0.131825904205330)
+cu = importlib.import_module("chaos-ultra_amd")
+
+
+def deep(width_plane, w=1920, h=1080, max_iter=50000, bailout_sq=256.0, rebase=True):
+    return cu.make_deep(CENTRE[0], CENTRE[1], width_plane, w, h, max_iter, bailout_sq, rebase=rebase)
+
+
+def mp_escape(cx, cy, max_iter, bailout_sq, prec=300):
+    """Direct iteration of z <- z^2 + c at `prec` bits; (count, |z|^2 at escape)."""
+    with mpmath.workprec(prec):
+        x = y = mpmath.mpf(0)
+        cx, cy = mpmath.mpf(cx), mpmath.mpf(cy)
+        for n in range(max_iter):
+            m = x * x + y * y
+            if m >= bailout_sq:
+                return n, m
+            x, y = x * x - y * y + cx, 2 * x * y + cy
+        return max_iter, None
+
+
+def test_dd_add_is_exact():
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        a, b = rng.standard_normal(2) * 10.0 ** rng.integers(-30, 30, 2)
+        hi, lo = O.dd_add(float(a), 0.0, float(b), 0.0)
+        assert Fraction(hi) + Fraction(lo) == Fraction(float(a)) + Fraction(float(b))
+        assert abs(lo) <= abs(hi) * 2.0 ** -52
+
+
+def test_reference_orbit_follows_300_bit_orbit():
+    orbit, M = O.reference_orbit(CENTRE[0], CENTRE[1], 50000, 256.0, 64)
+    with mpmath.workprec(300):
+        x = y = mpmath.mpf(0)
+        cx, cy = mpmath.mpf(CENTRE[0]), mpmath.mpf(CENTRE[1])
+        true_m = None
+        for n in range(50001):
+            err = max(abs(float(x) - orbit[n, 0]), abs(float(y) - orbit[n, 1])) if n <= M else 0
+            assert err <= 2.3e-16 * max(1.0, abs(float(x)), abs(float(y))), (n, err)
+            if x * x + y * y >= 256:
+                true_m = n
+                break
+            x, y = x * x - y * y + cx, 2 * x * y + cy
+    assert M == true_m == 3087
+    # the plain FP64 orbit of the same c leaves one step later
+    mag, esc = O.C.c_double(), O.C.c_int()
+    n64 = O.lib().or_escape_d(50000, 0.0, 0.0, CENTRE[0], CENTRE[1], 256.0, O.C.byref(mag),
+                              O.C.byref(esc))
+    assert esc.value == 1 and n64 == 3088
+
+
+def test_reference_orbit_storage():
+    o64, M64 = O.reference_orbit(CENTRE[0], CENTRE[1], 50000, 256.0, 64)
+    o32, M32 = O.reference_orbit(CENTRE[0], CENTRE[1], 50000, 256.0, 32)
+    assert M64 == M32
+    assert np.array_equal(o32[:, :2], o64[:, :2].astype(np.float32))
+    assert np.all(o64[:, 3] == 0) and o64[0].tolist() == [0, 0, 0, 0]
+    assert np.allclose(o64[:, 2], 0.5 * (o64[:, 0] ** 2 + o64[:, 1] ** 2), rtol=1e-15, atol=0)
+    # a short budget stops the orbit at max_iter
+    o, M = O.reference_orbit(CENTRE[0], CENTRE[1], 100, 256.0, 64)
+    assert M == 100 and np.array_equal(o, o64[:101])
+
+
+@pytest.mark.parametrize("width,n", [(1e-10, 12), (1e-20, 6), (1e-6, 3)])
+def test_dd_oracle_matches_300_bit_iteration(width, n):
+    d = deep(width)
+    g = np.load(os.path.join(GOLDEN, "deep_dd_subset.npz"))
+    idx = g["index"][:: len(g["index"]) // n][:n]
+    px, py = idx % 1920, idx // 1920
+    counts, mag, state, _ = O.deep_pixels(O.DEEP_DIRECT_DD, px, py, d)
+    for k in range(n):
+        ox, oy = O.C.c_double(), O.C.c_double()
+        O.lib().or_deep_offset(int(px[k]), int(py[k]), 1920, 1080, d.pixel_w, d.pixel_h,
+                               O.C.byref(ox), O.C.byref(oy))
+        with mpmath.workprec(300):
+            cx = mpmath.mpf(CENTRE[0]) + mpmath.mpf(ox.value)
+            cy = mpmath.mpf(CENTRE[1]) + mpmath.mpf(oy.value)
+        want, wmag = mp_escape(cx, cy, 50000, 256.0)
+        assert (counts[k] if state[k] == 1 else 50000) == want
+        if wmag is not None:
+            assert mag[k] == pytest.approx(float(wmag), rel=1e-9)
+
+
+def test_fixture_is_the_oracle():
+    g = np.load(os.path.join(GOLDEN, "deep_dd_subset.npz"))
+    idx = g["index"][::128]
+    for w in (1e-6, 1e-10, 1e-20):
+        counts, mag, state, _ = O.deep_pixels(O.DEEP_DIRECT_DD, idx % 1920, idx // 1920, deep(w))
+        assert np.array_equal(counts, g[f"counts_{w:.0e}"][::128])
+        sm = np.where(state == 1, counts + 1.0 - np.log2(np.log(np.where(state == 1, mag, 256.0))
+                                                        / math.log(256.0)), 0.0)
+        assert np.allclose(sm, g[f"smooth_{w:.0e}"][::128], rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_zero_delta_reproduces_reference_escape(precision):
+    orbit, M = O.reference_orbit(CENTRE[0], CENTRE[1], 50000, 256.0, precision)
+    mag = O.C.c_double() if precision == 64 else O.C.c_float()
+    st = O.C.c_int()
+    f = O.lib().or_perturb_pixel_d if precision == 64 else O.lib().or_perturb_pixel_f
+    n = f(orbit.ctypes.data_as(O.C.c_void_p), M, 0.0, 0.0, 50000, 256.0, 1, 1e-6, O.C.byref(mag),
+          O.C.byref(st))
+    assert (n, st.value) == (M, 1)
+
+
+def test_fp64_perturbation_matches_direct_at_shallow_zoom():
+    d = cu.make_deep(-0.6, 0.0, 2.5, 160, 90, 500, 256.0)
+    d.centre_x, d.centre_y = -0.6, 0.0
+    orbit, M = O.reference_orbit(-0.6, 0.0, 500, 256.0, 64)
+    ys, xs = np.mgrid[0:90, 0:160]
+    pc, _, ps, _ = O.deep_pixels(O.DEEP_PERTURB_F64, xs.ravel(), ys.ravel(), d, orbit, M)
+    dc, _, ds, _ = O.deep_pixels(O.DEEP_DIRECT_F64, xs.ravel(), ys.ravel(), d)
+    assert np.mean(pc != dc) < 0.005
+    assert not np.any(ps == 3)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_glitches_resolved_by_secondary_reference(precision):
+    method = O.DEEP_PERTURB_F64 if precision == 64 else O.DEEP_PERTURB_F32
+    d = deep(1e-10, 192, 108, 50000, rebase=False)
+    orbit, M = O.reference_orbit(CENTRE[0], CENTRE[1], d.max_iter, 256.0, precision)
+    ys, xs = np.mgrid[0:108, 0:192]
+    px, py = xs.ravel(), ys.ravel()
+    counts, _, state, _ = O.deep_pixels(method, px, py, d, orbit, M)
+    glitched = state == 3
+    assert glitched.sum() > 100
+    # secondary reference: the glitched pixel nearest the glitched centroid
+    gx, gy = px[glitched], py[glitched]
+    k = np.argmin((gx - gx.mean()) ** 2 + (gy - gy.mean()) ** 2)
+    ox, oy = O.C.c_double(), O.C.c_double()
+    O.lib().or_deep_offset(int(gx[k]), int(gy[k]), 192, 108, d.pixel_w, d.pixel_h, O.C.byref(ox),
+                           O.C.byref(oy))
+    rx, ry = O.dd_add(CENTRE[0], 0.0, ox.value), O.dd_add(CENTRE[1], 0.0, oy.value)
+    orbit2, M2 = O.reference_orbit(rx[0], ry[0], d.max_iter, 256.0, precision, lo=(rx[1], ry[1]))
+    c2, _, s2, _ = O.deep_pixels(method, gx, gy, d, orbit2, M2, ref=(ox.value, oy.value))
+    assert (s2 == 3).sum() < glitched.sum() / 4
+    # re-rendered pixels agree with the rebasing render (FP64 deltas: nearly all)
+    d.flags = 1
+    cr, _, sr, _ = O.deep_pixels(method, gx, gy, d, orbit, M)
+    ok = s2 != 3
+    if precision == 64:
+        assert np.mean(c2[ok] != cr[ok]) < 0.02
