@@ -9,7 +9,10 @@ Inputs (tile descriptors, palette) are resident on the device before timing.
 value = Giga-iterations/s over the timed steps (iterations executed; an
 interior sample counts maxIter).
 
-Also reported: the roofline of the dominant kernel (pass 1 of the pipeline:
+Also reported (variants): the 256-tile uneven batch (autotuned, captured
+replay), the 1920x1080 deep-zoom frames (FP32 and FP64-delta perturbation and
+direct FP64 with their mismatch against the double-double oracle subset), and
+the roofline of the dominant kernel (pass 1 of the pipeline:
 8 algorithmic FP64 FLOPs per iteration it executes, against the 78.6 TFLOP/s
 FP64 vector peak and the 39.3 TFLOP/s add/mul-only ceiling), the thread-per-
 sample grid kernel and the FP32 pipeline on the same frame, and the oracle's CPU
@@ -140,6 +143,88 @@ def timed(cu, run, steps: int, warmup: int):
     return dt, total, first
 
 
+def uneven_leg(cu, engine):
+    """The 256-tile uneven batch: default pipeline tune vs the autotuner's pick."""
+    wl = importlib.import_module("chaos-ultra_amd.workloads")
+    at = importlib.import_module("chaos-ultra_amd.autotune")
+    t0 = time.perf_counter()
+    tiles, info = wl.uneven_tiles(cu, 256)
+    gen_s = time.perf_counter() - t0
+    res = at.tune_batch(cu, engine, tiles, repeats=3, warmup=1)
+    frame = cu.Frame(engine, tiles, 64, -1)
+    out = {"tiles": len(tiles), "pixels": int(sum(i["side"] ** 2 for i in info)),
+           "generate_s": round(gen_s, 1)}
+    for name, tn in (("default", cu.FrameTune(16, 16, 5, 0)),
+                     ("tuned_pipeline", cu.FrameTune(res["pipeline"]["best"]["block_steps"],
+                                                     res["pipeline"]["best"]["handoff_lanes"],
+                                                     res["pipeline"]["best"]["tail_passes"],
+                                                     res["pipeline"]["best"]["wave_shape"]))):
+        ms = []
+        for _ in range(4):
+            frame.render(tn)
+            ms.append(frame.last_ms()[1])
+        it = frame.iterations()
+        m = float(np.median(ms[1:]))
+        out[name] = {"render_ms": round(m, 3), "giga_iter_s": round(it / m / 1e6, 1),
+                     "iterations": int(it)}
+    frame.capture(tn)
+    ms = []
+    for _ in range(4):
+        frame.replay()
+        ms.append(frame.last_ms()[1])
+    out["tuned_pipeline"]["graph_replay_ms"] = round(float(np.median(ms[1:])), 3)
+    frame.close()
+    if res["engine"] == "persistent":  # time the winner on its own
+        batch = at.PersistentBatch(cu, tiles)
+        p = res["tune"]
+        tn = cu.Tune(p["block_steps"], p["waves_per_cu"], p["chunk"], p["refill_lanes"])
+        ms = [batch.render(engine, tn) for _ in range(4)]
+        it = batch.iters.total()
+        m = float(np.median(ms[1:]))
+        out["tuned_persistent"] = {"render_ms": round(m, 3), "giga_iter_s": round(it / m / 1e6, 1),
+                                   "iterations": int(it)}
+    out["autotune"] = {"engine": res["engine"], "tune": res["tune"], "ms": round(res["ms"], 3),
+                       "pipeline_best": res["pipeline"]["best"],
+                       "persistent_best": res.get("persistent", {}).get("best"),
+                       "persistent_ms": round(res.get("persistent", {}).get("ms", float("nan")), 3),
+                       "candidates": len(res["pipeline"]["table"]) +
+                                     len(res.get("persistent", {}).get("table", []))}
+    return out
+
+
+def deep_leg(cu, engine):
+    """1920x1080 deep-zoom frames (maxIter 50000): FP64/FP32-delta perturbation and
+    direct FP64, rate and mismatch against the committed double-double subset."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "deep_dd_subset.npz"))
+    idx = g["index"]
+    out = {}
+    for w in (1e-6, 1e-10, 1e-20):
+        row = {}
+        for name, method in (("perturb_f64", cu.DEEP_PERTURB_F64), ("perturb_f32", cu.DEEP_PERTURB_F32),
+                             ("direct_f64", cu.DEEP_DIRECT_F64)):
+            if method == cu.DEEP_DIRECT_F64 and w < 1e-13:
+                continue  # below the FP64 ulp of the centre
+            d = cu.make_deep(-0.743643887037151, 0.131825904205330, w, 1920, 1080, 50000, 256.0)
+            f = cu.DeepFrame(engine, d, method)
+            ms = []
+            for _ in range(3):
+                f.render()
+                ms.append(f.report().total_ms)
+            rep = f.report()
+            counts = f.counts().ravel()[idx]
+            f.close()
+            m = float(np.median(ms))
+            row[name] = {"frame_ms": round(m, 3), "giga_iter_s": round(rep.iterations / m / 1e6, 1),
+                         "mismatch_vs_dd_pct": round(100 * float(np.mean(counts != g[f"counts_{w:.0e}"])), 3)}
+            if method != cu.DEEP_DIRECT_F64:
+                row[name]["orbit_ms"] = round(rep.orbit_ms, 3)
+        if "direct_f64" in row:
+            row["perturb_f32_over_direct_f64"] = round(row["perturb_f32"]["giga_iter_s"] /
+                                                       row["direct_f64"]["giga_iter_s"], 3)
+        out[f"width_{w:.0e}"] = row
+    return out
+
+
 def read_traffic():
     """HBM bytes per pass-1 launch from the committed counter summary, if present."""
     try:
@@ -185,7 +270,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-variants", action="store_true", help="skip the grid / FP32 variants")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the grid / FP32 variants and the uneven-tile and deep-zoom legs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (a plain copy "
                          "of known size that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -230,6 +316,14 @@ def main():
                     "ulp_f32(0.74) 5.96e-8): a throughput figure, not an image"}
         log(f"[bench] FP32 pipeline: {variants['fp32_pipeline']}")
         r32.close()
+        log("[bench] uneven tiles (256, autotuned)")
+        variants["uneven_tiles_256"] = uneven_leg(cu, engine)
+        u = variants["uneven_tiles_256"]
+        log(f"[bench] uneven tiles: pipeline {u['tuned_pipeline']}, winner {u['autotune']['engine']} "
+            f"{u.get('tuned_persistent', u['tuned_pipeline'])}")
+        log("[bench] deep zoom 1920x1080")
+        variants["deep_zoom_1080p"] = deep_leg(cu, engine)
+        log(f"[bench] deep zoom: {json.dumps(variants['deep_zoom_1080p'])}")
 
     result = {
         "metric": METRIC,

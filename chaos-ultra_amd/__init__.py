@@ -356,6 +356,15 @@ class Frame:
                  palette.ptr if palette else None, palette_length, rgba.ptr if rgba else None,
                  stream.handle if stream else None)
 
+    def capture(self, tune: FrameTune | None = None, palette: DeviceBuffer | None = None,
+                palette_length: int = 0, rgba: DeviceBuffer | None = None) -> None:
+        """Capture one render into a HIP graph (replayed by replay())."""
+        abi.call("cu_frame_capture", self.handle, C.byref(tune) if tune else None,
+                 palette.ptr if palette else None, palette_length, rgba.ptr if rgba else None)
+
+    def replay(self, stream: Stream | None = None) -> None:
+        abi.call("cu_frame_replay", self.handle, stream.handle if stream else None)
+
     def counts(self) -> np.ndarray:
         out = np.empty(self.total_pixels, dtype=np.uint32)
         abi.call("cu_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), C.c_void_p(self.d_counts),

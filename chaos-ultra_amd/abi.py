@@ -127,7 +127,7 @@ class Tune(C.Structure):
 class FrameTune(C.Structure):
     """cu_frame_tune"""
     _fields_ = [("block_steps", C.c_uint32), ("handoff_lanes", C.c_uint32),
-                ("tail_passes", C.c_uint32), ("reserved", C.c_uint32)]
+                ("tail_passes", C.c_uint32), ("wave_shape", C.c_uint32)]
 
 
 class DeepDesc(C.Structure):
@@ -230,6 +230,8 @@ SIGNATURES = {
     "cu_frame_outputs": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_U64)]),
     "cu_frame_iterations": (_I, [_P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "cu_frame_last_ms": (_I, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "cu_frame_capture": (_I, [_P, C.POINTER(FrameTune), _P, _U32, _P]),
+    "cu_frame_replay": (_I, [_P, _P]),
     "cu_engine_configure": (_I, [_P, C.POINTER(Tune)]),
     "cu_deep_create": (_I, [_P, C.POINTER(DeepDesc), _I, C.POINTER(_P)]),
     "cu_deep_destroy": (_I, [_P]),

@@ -1,0 +1,124 @@
+"""Autotuner for batched tile renders: picks the engine and its launch shape by
+measuring them on the batch itself.
+
+Candidates:
+  * the straggler-compaction frame pipeline (cu_frame): pass-1 wave shape (the
+    pixel block a wave covers: 8x8 .. 64x1 pixels at 1 spp), unchecked block
+    length, hand-off threshold (lanes still running when a wave hands its
+    stragglers to the queue) and number of tail passes;
+  * the persistent lane-refill kernel (cu_escape_tiles_persistent): block
+    length, resident waves per CU, pixels taken per queue grab and the number
+    of idle lanes that trigger a refill.
+
+The search is coordinate descent from a default point, one knob at a time in
+the order above, each candidate timed with HIP events on its stream (median of
+`repeats` after `warmup` renders).  Every candidate renders the same pixels;
+the caller can check their outputs against each other.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+PIPELINE_SPACE = {
+    "wave_shape": (0, 1, 2, 3),
+    "block_steps": (16, 8),
+    "handoff_lanes": (8, 16, 24, 32),
+    "tail_passes": (3, 5, 7),
+}
+PIPELINE_START = {"block_steps": 16, "handoff_lanes": 16, "tail_passes": 5, "wave_shape": 0}
+
+PERSISTENT_SPACE = {
+    "block_steps": (16, 8, 32),
+    "waves_per_cu": (32, 16, 24),
+    "chunk": (256, 64, 1024),
+    "refill_lanes": (16, 8, 32),
+}
+PERSISTENT_START = {"block_steps": 16, "waves_per_cu": 32, "chunk": 256, "refill_lanes": 16}
+
+
+def _median_ms(fn, repeats, warmup):
+    for _ in range(warmup):
+        fn()
+    ts = [fn() for _ in range(repeats)]
+    return float(np.median(ts))
+
+
+def _descend(space, start, measure, log):
+    best = dict(start)
+    best_ms = measure(best)
+    table = [(dict(best), best_ms)]
+    for knob, values in space.items():
+        for v in values:
+            if v == best[knob]:
+                continue
+            cand = dict(best)
+            cand[knob] = v
+            ms = measure(cand)
+            table.append((cand, ms))
+            if log:
+                log(f"[autotune] {cand} {ms:.3f} ms")
+            if ms < best_ms:
+                best, best_ms = cand, ms
+    return best, best_ms, table
+
+
+class PersistentBatch:
+    """Device copy of a tile batch laid out back to back for the persistent kernel."""
+
+    def __init__(self, cu, tiles):
+        self.cu = cu
+        arr = (cu.TileDesc * len(tiles))()
+        offset = 0
+        for k, t in enumerate(tiles):
+            C.memmove(C.byref(arr[k]), C.byref(t), C.sizeof(t))
+            arr[k].work_offset = offset
+            arr[k].out_offset = offset
+            arr[k].out_pitch = t.width
+            offset += t.width * t.height
+        self.n, self.total = len(tiles), offset
+        self.d_tiles = cu.DeviceBuffer(C.sizeof(arr))
+        cu.abi.call("cu_memcpy_htod", self.d_tiles.ptr, C.cast(arr, C.c_void_p), C.sizeof(arr))
+        self.counts = cu.DeviceBuffer(offset * 4)
+        self.smooth = cu.DeviceBuffer(offset * 8)
+        self.iters = cu.IterationCounter()
+        self.queue = cu.DeviceBuffer(8)
+        self.variant = 1 if all(cu.tile_fused_safe(t, 64) for t in tiles) else 0
+
+    def render(self, engine, tune, precision=64) -> float:
+        self.iters.clear()
+        engine.escape_tiles_persistent(precision, self.variant, self.d_tiles, self.n, self.total,
+                                       self.counts, self.smooth, self.iters, self.queue, tune)
+        return engine.last_kernel_ms()
+
+
+def tune_batch(cu, engine, tiles, precision=64, repeats=3, warmup=1, persistent=True, log=None):
+    """Measure both engines on `tiles`; returns a dict with the winner and the tables."""
+    frame = cu.Frame(engine, tiles, precision, -1)
+
+    def measure_pipeline(p):
+        tn = cu.FrameTune(p["block_steps"], p["handoff_lanes"], p["tail_passes"], p["wave_shape"])
+
+        def one():
+            frame.render(tn)
+            return frame.last_ms()[1]
+        return _median_ms(one, repeats, warmup)
+
+    pbest, pms, ptable = _descend(PIPELINE_SPACE, PIPELINE_START, measure_pipeline, log)
+    result = {"pipeline": {"best": pbest, "ms": pms, "table": ptable}}
+    frame.close()
+    best = ("pipeline", pbest, pms)
+    if persistent and precision == 64:
+        batch = PersistentBatch(cu, tiles)
+
+        def measure_persistent(p):
+            tn = cu.Tune(p["block_steps"], p["waves_per_cu"], p["chunk"], p["refill_lanes"])
+            return _median_ms(lambda: batch.render(engine, tn, precision), repeats, warmup)
+
+        qbest, qms, qtable = _descend(PERSISTENT_SPACE, PERSISTENT_START, measure_persistent, log)
+        result["persistent"] = {"best": qbest, "ms": qms, "table": qtable}
+        if qms < pms:
+            best = ("persistent", qbest, qms)
+    result["engine"], result["tune"], result["ms"] = best
+    return result

@@ -31,16 +31,21 @@ struct PipeArgs {
     uint32_t handoff;
     uint32_t subq_capacity;
     uint32_t palette_length;
+    uint32_t wave_shape;
+    uint32_t pad;
 };
-static_assert(sizeof(PipeArgs) == 112, "PipeArgs layout");
+static_assert(sizeof(PipeArgs) == 120, "PipeArgs layout");
 
 constexpr uint32_t kMaxHandoff = 32;   // records a wave may push
+constexpr uint32_t kWaveShapes = 4;    // pixel-block shapes of a pass-1 wave
 constexpr uint32_t kSubQueues = 1024;  // sub-queues per straggler queue (kernel constant)
 
-void wave_pixel_block(uint32_t spp, uint32_t &bw, uint32_t &bh) {
+// The bw x bh pixel block of a pass-1 wave (bw * bh = 64 / spp): shape 0 is the
+// near-square block, each further shape twice as wide (capped at one row).
+void wave_pixel_block(uint32_t spp, uint32_t shape, uint32_t &bw, uint32_t &bh) {
     uint32_t ppw = 64 / spp, lg = 0;
     while ((1u << (lg + 1)) <= ppw) ++lg;
-    bw = 1u << ((lg + 1) >> 1);
+    bw = std::min(ppw, 1u << (((lg + 1) >> 1) + shape));
     bh = ppw / bw;
 }
 
@@ -52,7 +57,7 @@ struct cu_frame_s {
     bool fused = false;
     std::vector<cu_tile_desc> tiles;
     uint64_t total_pixels = 0, total_samples = 0;
-    uint32_t total_blocks = 0;
+    uint32_t total_blocks[kWaveShapes] = {0, 0, 0, 0};
     uint32_t subq_cap = 0;  // records per sub-queue
     void *d_tiles = nullptr, *d_block_offset = nullptr, *d_sample_offset = nullptr;
     void *d_smooth_c = nullptr;  // f64 per tile: log2(log2(bailout_sq))
@@ -65,6 +70,9 @@ struct cu_frame_s {
     hipEvent_t ev_start = nullptr, ev_pass1 = nullptr, ev_end = nullptr;
     hipStream_t last_stream = nullptr;
     bool rendered = false;
+    bool replayed = false;                 // last render was a graph replay (no pass-1 event)
+    hipGraph_t graph = nullptr;            // captured render (cu_frame_capture)
+    hipGraphExec_t graph_exec = nullptr;
 };
 
 namespace cu {
@@ -79,11 +87,98 @@ void free_frame(cu_frame f) {
         if (p) (void)hipFree(p);
     for (hipEvent_t e : {f->ev_start, f->ev_pass1, f->ev_end})
         if (e) (void)hipEventDestroy(e);
+    if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
+    if (f->graph) (void)hipGraphDestroy(f->graph);
     delete f;
 }
 
 void fill(cu_frame f, cu_stream s, void *p, uint32_t value, uint64_t words) {
     fill_u32(f->engine, s, p, value, words);
+}
+
+cu_frame_tune checked_tune(const cu_frame_tune *tune, const uint32_t *d_palette,
+                           uint32_t palette_length, const uint32_t *d_rgba) {
+    cu_frame_tune tn{16, 16, 5, 0};
+    if (tune) tn = *tune;
+    if (tn.block_steps != 8 && tn.block_steps != 16)
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "block_steps must be 8 or 16");
+    if (tn.handoff_lanes == 0 || tn.handoff_lanes > kMaxHandoff)
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "handoff_lanes must be in [1, 32]");
+    if (tn.tail_passes == 0 || tn.tail_passes > 8)
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "tail_passes must be in [1, 8]");
+    if (tn.wave_shape >= kWaveShapes) fail(CU_ERR_ILLEGAL_ARGUMENT, "wave_shape must be in [0, 3]");
+    if ((d_rgba == nullptr) != (d_palette == nullptr))
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "palette and rgba go together");
+    if (d_palette && palette_length == 0) fail(CU_ERR_ILLEGAL_ARGUMENT, "empty palette");
+    return tn;
+}
+
+// All launches of one render on `stream`: clears, pass 1, the tail passes and
+// the combine of pixels with queued samples.  `events` records the timing
+// events (not while capturing a graph).
+void enqueue_render(cu_frame f, const cu_frame_tune &tn, const uint32_t *d_palette,
+                    uint32_t palette_length, uint32_t *d_rgba, cu_stream stream, bool events) {
+    hipStream_t st = stream ? stream->stream : nullptr;
+    const std::string suffix = std::string(real_name(f->precision)) +
+                               (f->fused ? "Fused" : "Exact") + "K" +
+                               std::to_string(tn.block_steps);
+    cu_function pass1 = function(f->engine, "escapePass1" + suffix);
+    cu_function passq = function(f->engine, "escapePassQ" + suffix);
+    cu_function combine = function(f->engine, "combinePending");
+    uint32_t *counters = (uint32_t *)f->d_counters;
+    const uint32_t n_tiles = (uint32_t)f->tiles.size();
+
+    fill(f, stream, f->d_iters, 0, 2 * CU_ITER_SLOTS * 2);
+    fill(f, stream, counters, 0, kSubQueues);
+    if (events) check(hipEventRecord(f->ev_start, st), "hipEventRecord");
+
+    PipeArgs a{(const cu_tile_desc *)f->d_tiles,
+               (const uint32_t *)f->d_block_offset + tn.wave_shape * (n_tiles + 1),
+               (const uint64_t *)f->d_sample_offset,
+               (const double *)f->d_smooth_c,
+               (uint32_t *)f->d_sample_count,
+               (double *)f->d_sample_smooth,
+               (unsigned long long *)f->d_iters,
+               (uint32_t *)f->d_counts,
+               (double *)f->d_smooth,
+               d_rgba,
+               d_palette,
+               (unsigned char *)f->d_pending,
+               n_tiles,
+               tn.handoff_lanes,
+               f->subq_cap,
+               d_palette ? palette_length : 0u,
+               tn.wave_shape,
+               0u};
+    {
+        uint32_t total_blocks = f->total_blocks[tn.wave_shape];
+        void *q = f->d_queue[0];
+        uint32_t *c = counters;
+        void *params[] = {&a, &total_blocks, &q, &c};
+        launch(f->engine, pass1, dim3((total_blocks + 3) / 4), dim3(256), params, stream, false);
+    }
+    if (events) check(hipEventRecord(f->ev_pass1, st), "hipEventRecord");
+    const uint32_t cus = (uint32_t)std::max(1, compute_units());
+    for (uint32_t p = 1; p <= tn.tail_passes; ++p) {
+        const uint32_t in = (p - 1) & 1u, outq = p & 1u;
+        fill(f, stream, counters + outq * kSubQueues, 0, kSubQueues);
+        PipeArgs ap = a;
+        ap.handoff = p == tn.tail_passes ? 0u : tn.handoff_lanes;
+        ap.iters = (unsigned long long *)f->d_iters + CU_ITER_SLOTS;
+        const void *qin = f->d_queue[in];
+        const uint32_t *cin = counters + in * kSubQueues;
+        void *qout = f->d_queue[outq];
+        uint32_t *cout = counters + outq * kSubQueues;
+        void *params[] = {&ap, &qin, &cin, &qout, &cout};
+        launch(f->engine, passq, dim3(cus * 8), dim3(256), params, stream, false);
+    }
+    {
+        uint64_t total = f->total_pixels;
+        void *params[] = {&a, &total};
+        unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, (uint64_t)cus * 32);
+        launch(f->engine, combine, dim3(blocks), dim3(256), params, stream, false);
+    }
+    if (events) check(hipEventRecord(f->ev_end, st), "hipEventRecord");
 }
 
 }  // namespace
@@ -106,11 +201,11 @@ int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, in
         f->engine = e;
         f->precision = precision;
         f->tiles.assign(tiles, tiles + n_tiles);
-        std::vector<uint32_t> block_offset(n_tiles + 1, 0);
+        std::vector<uint32_t> block_offset(kWaveShapes * (n_tiles + 1), 0);
         std::vector<uint64_t> sample_offset(n_tiles, 0);
         std::vector<double> smooth_c(n_tiles, 0.0);
         bool all_safe = true;
-        uint64_t blocks = 0, pixels = 0, samples = 0;
+        uint64_t blocks[kWaveShapes] = {0, 0, 0, 0}, pixels = 0, samples = 0;
         for (uint32_t k = 0; k < n_tiles; ++k) {
             cu_tile_desc &t = f->tiles[k];
             t.out_pitch = t.width;
@@ -119,10 +214,12 @@ int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, in
                 fail(CU_ERR_ILLEGAL_ARGUMENT, "frame tiles need spp a power of two <= 64");
             if ((uint64_t)t.width * t.height * t.spp >= (1ull << 32))
                 fail(CU_ERR_ILLEGAL_ARGUMENT, "tile holds 2^32 samples or more");
-            uint32_t bw, bh;
-            wave_pixel_block(t.spp, bw, bh);
-            block_offset[k] = (uint32_t)blocks;
-            blocks += (uint64_t)((t.width + bw - 1) / bw) * ((t.height + bh - 1) / bh);
+            for (uint32_t w = 0; w < kWaveShapes; ++w) {
+                uint32_t bw, bh;
+                wave_pixel_block(t.spp, w, bw, bh);
+                block_offset[w * (n_tiles + 1) + k] = (uint32_t)blocks[w];
+                blocks[w] += (uint64_t)((t.width + bw - 1) / bw) * ((t.height + bh - 1) / bh);
+            }
             sample_offset[k] = samples;
             t.work_offset = pixels;
             t.out_offset = pixels;
@@ -131,20 +228,24 @@ int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, in
             smooth_c[k] = smooth_constant(t.bailout_sq);
             all_safe = all_safe && fused_safe(t, precision);
         }
-        if (blocks >= (1ull << 31)) fail(CU_ERR_ILLEGAL_ARGUMENT, "frame too large");
-        block_offset[n_tiles] = (uint32_t)blocks;
+        uint64_t max_blocks = 0;
+        for (uint32_t w = 0; w < kWaveShapes; ++w) {
+            if (blocks[w] >= (1ull << 31)) fail(CU_ERR_ILLEGAL_ARGUMENT, "frame too large");
+            block_offset[w * (n_tiles + 1) + n_tiles] = (uint32_t)blocks[w];
+            f->total_blocks[w] = (uint32_t)blocks[w];
+            max_blocks = std::max(max_blocks, blocks[w]);
+        }
         f->fused = variant == 1 || (variant == -1 && all_safe);
-        f->total_blocks = (uint32_t)blocks;
         f->total_pixels = pixels;
         f->total_samples = samples;
         // pass 1 sends wave w to sub-queue w % kSubQueues; a later pass sends a
         // sub-queue's stragglers back to the same sub-queue, never more than it read
-        f->subq_cap = (uint32_t)((blocks + kSubQueues - 1) / kSubQueues) * kMaxHandoff;
+        f->subq_cap = (uint32_t)((max_blocks + kSubQueues - 1) / kSubQueues) * kMaxHandoff;
         const size_t queue_records = (size_t)kSubQueues * f->subq_cap;
         const size_t rec = precision == 64 ? 32 : 24;
         ensure_device();
         f->d_tiles = dalloc(n_tiles * sizeof(cu_tile_desc));
-        f->d_block_offset = dalloc((n_tiles + 1) * sizeof(uint32_t));
+        f->d_block_offset = dalloc(block_offset.size() * sizeof(uint32_t));
         f->d_sample_offset = dalloc(n_tiles * sizeof(uint64_t));
         f->d_smooth_c = dalloc(n_tiles * sizeof(double));
         f->d_sample_count = dalloc(samples * 4);
@@ -185,79 +286,65 @@ int cu_frame_render(cu_frame f, const cu_frame_tune *tune, const uint32_t *d_pal
                     uint32_t palette_length, uint32_t *d_rgba, cu_stream stream) {
     return guarded([&] {
         if (!f) fail(CU_ERR_ILLEGAL_ARGUMENT, "frame is NULL");
-        cu_frame_tune tn{16, 16, 3, 0};
-        if (tune) tn = *tune;
-        if (tn.block_steps != 8 && tn.block_steps != 16)
-            fail(CU_ERR_ILLEGAL_ARGUMENT, "block_steps must be 8 or 16");
-        if (tn.handoff_lanes == 0 || tn.handoff_lanes > kMaxHandoff)
-            fail(CU_ERR_ILLEGAL_ARGUMENT, "handoff_lanes must be in [1, 32]");
-        if (tn.tail_passes == 0 || tn.tail_passes > 8)
-            fail(CU_ERR_ILLEGAL_ARGUMENT, "tail_passes must be in [1, 8]");
-        if ((d_rgba == nullptr) != (d_palette == nullptr))
-            fail(CU_ERR_ILLEGAL_ARGUMENT, "palette and rgba go together");
-        if (d_palette && palette_length == 0) fail(CU_ERR_ILLEGAL_ARGUMENT, "empty palette");
+        const cu_frame_tune tn = checked_tune(tune, d_palette, palette_length, d_rgba);
+        std::lock_guard<std::mutex> lock(f->engine->mutex);
+        ensure_device();
+        enqueue_render(f, tn, d_palette, palette_length, d_rgba, stream, true);
+        f->last_stream = stream ? stream->stream : nullptr;
+        f->rendered = true;
+        f->replayed = false;
+    });
+}
+
+int cu_frame_capture(cu_frame f, const cu_frame_tune *tune, const uint32_t *d_palette,
+                     uint32_t palette_length, uint32_t *d_rgba) {
+    return guarded([&] {
+        if (!f) fail(CU_ERR_ILLEGAL_ARGUMENT, "frame is NULL");
+        const cu_frame_tune tn = checked_tune(tune, d_palette, palette_length, d_rgba);
+        std::lock_guard<std::mutex> lock(f->engine->mutex);
+        ensure_device();
+        hipStream_t cs = nullptr;
+        check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        cu_stream_s wrap{cs};
+        hipGraph_t g = nullptr;
+        try {
+            check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                enqueue_render(f, tn, d_palette, palette_length, d_rgba, &wrap, false);
+            } catch (...) {
+                (void)hipStreamEndCapture(cs, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            check(hipStreamEndCapture(cs, &g), "hipStreamEndCapture");
+            hipGraphExec_t exec = nullptr;
+            check(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+            if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
+            if (f->graph) (void)hipGraphDestroy(f->graph);
+            f->graph = g;
+            f->graph_exec = exec;
+        } catch (...) {
+            if (g && g != f->graph) (void)hipGraphDestroy(g);
+            (void)hipStreamDestroy(cs);
+            throw;
+        }
+        check(hipStreamDestroy(cs), "hipStreamDestroy");
+    });
+}
+
+int cu_frame_replay(cu_frame f, cu_stream stream) {
+    return guarded([&] {
+        if (!f) fail(CU_ERR_ILLEGAL_ARGUMENT, "frame is NULL");
+        if (!f->graph_exec) fail(CU_ERR_ILLEGAL_STATE, "frame has no captured render");
         std::lock_guard<std::mutex> lock(f->engine->mutex);
         ensure_device();
         hipStream_t st = stream ? stream->stream : nullptr;
-        const std::string suffix = std::string(real_name(f->precision)) +
-                                   (f->fused ? "Fused" : "Exact") + "K" +
-                                   std::to_string(tn.block_steps);
-        cu_function pass1 = function(f->engine, "escapePass1" + suffix);
-        cu_function passq = function(f->engine, "escapePassQ" + suffix);
-        cu_function combine = function(f->engine, "combinePending");
-        uint32_t *counters = (uint32_t *)f->d_counters;
-
-        fill(f, stream, f->d_iters, 0, 2 * CU_ITER_SLOTS * 2);
-        fill(f, stream, counters, 0, kSubQueues);
         check(hipEventRecord(f->ev_start, st), "hipEventRecord");
-
-        PipeArgs a{(const cu_tile_desc *)f->d_tiles,
-                   (const uint32_t *)f->d_block_offset,
-                   (const uint64_t *)f->d_sample_offset,
-                   (const double *)f->d_smooth_c,
-                   (uint32_t *)f->d_sample_count,
-                   (double *)f->d_sample_smooth,
-                   (unsigned long long *)f->d_iters,
-                   (uint32_t *)f->d_counts,
-                   (double *)f->d_smooth,
-                   d_rgba,
-                   d_palette,
-                   (unsigned char *)f->d_pending,
-                   (uint32_t)f->tiles.size(),
-                   tn.handoff_lanes,
-                   f->subq_cap,
-                   d_palette ? palette_length : 0u};
-        {
-            uint32_t total_blocks = f->total_blocks;
-            void *q = f->d_queue[0];
-            uint32_t *c = counters;
-            void *params[] = {&a, &total_blocks, &q, &c};
-            launch(f->engine, pass1, dim3((total_blocks + 3) / 4), dim3(256), params, stream, false);
-        }
-        check(hipEventRecord(f->ev_pass1, st), "hipEventRecord");
-        const uint32_t cus = (uint32_t)std::max(1, compute_units());
-        for (uint32_t p = 1; p <= tn.tail_passes; ++p) {
-            const uint32_t in = (p - 1) & 1u, outq = p & 1u;
-            fill(f, stream, counters + outq * kSubQueues, 0, kSubQueues);
-            PipeArgs ap = a;
-            ap.handoff = p == tn.tail_passes ? 0u : tn.handoff_lanes;
-            ap.iters = (unsigned long long *)f->d_iters + CU_ITER_SLOTS;
-            const void *qin = f->d_queue[in];
-            const uint32_t *cin = counters + in * kSubQueues;
-            void *qout = f->d_queue[outq];
-            uint32_t *cout = counters + outq * kSubQueues;
-            void *params[] = {&ap, &qin, &cin, &qout, &cout};
-            launch(f->engine, passq, dim3(cus * 8), dim3(256), params, stream, false);
-        }
-        {
-            uint64_t total = f->total_pixels;
-            void *params[] = {&a, &total};
-            unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, (uint64_t)cus * 32);
-            launch(f->engine, combine, dim3(blocks), dim3(256), params, stream, false);
-        }
+        check(hipGraphLaunch(f->graph_exec, st), "hipGraphLaunch");
         check(hipEventRecord(f->ev_end, st), "hipEventRecord");
         f->last_stream = st;
         f->rendered = true;
+        f->replayed = true;
     });
 }
 
@@ -291,7 +378,10 @@ int cu_frame_last_ms(cu_frame f, float *pass1_ms, float *total_ms) {
         if (!f->rendered) fail(CU_ERR_ILLEGAL_STATE, "frame has not been rendered");
         ensure_device();
         check(hipEventSynchronize(f->ev_end), "hipEventSynchronize");
-        if (pass1_ms) check(hipEventElapsedTime(pass1_ms, f->ev_start, f->ev_pass1), "hipEventElapsedTime");
+        if (pass1_ms) {
+            if (f->replayed) *pass1_ms = 0.0f;  // a graph replay records no pass-1 event
+            else check(hipEventElapsedTime(pass1_ms, f->ev_start, f->ev_pass1), "hipEventElapsedTime");
+        }
         if (total_ms) check(hipEventElapsedTime(total_ms, f->ev_start, f->ev_end), "hipEventElapsedTime");
     });
 }

@@ -105,10 +105,12 @@ __device__ void escape_tile(const cu_tile_desc &t, double smooth_c, uint *counts
 // run on all lanes at once.  The spp lanes of a pixel sum their counts and
 // smooth values in sample order (lane s==0 adds lanes 1..spp-1 in turn), the
 // same order as a sequential loop.  Block: 256 threads = 4 waves in 2x2.
-__device__ __forceinline__ void wave_pixel_block(uint spp, uint &bw, uint &bh) {
+// shape 0 is the near-square block; each further shape is twice as wide (capped
+// at one row of 64/spp pixels).
+__device__ __forceinline__ void wave_pixel_block(uint spp, uint &bw, uint &bh, uint shape = 0) {
     const uint ppw = 64u / spp;
     const uint lg = 31u - __clz(ppw);
-    bw = 1u << ((lg + 1u) >> 1);
+    bw = min(ppw, 1u << (((lg + 1u) >> 1) + shape));
     bh = ppw / bw;
 }
 

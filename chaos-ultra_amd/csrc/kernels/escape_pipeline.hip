@@ -51,8 +51,10 @@ struct PipeArgs {
     uint handoff;                   // hand off when running lanes <= handoff (0: run to completion)
     uint subq_capacity;             // records per sub-queue
     uint palette_length;
+    uint wave_shape;                // pass-1 pixel block of a wave (wave_pixel_block)
+    uint pad;
 };
-static_assert(sizeof(PipeArgs) == 112, "PipeArgs layout");
+static_assert(sizeof(PipeArgs) == 120, "PipeArgs layout");
 
 // The straggler queue is kSubQueues sub-queues, each a region of subq_capacity
 // records with its own fill counter: the wave (or queue chunk) number picks the
@@ -193,7 +195,7 @@ __device__ void pass1(const PipeArgs a, uint total_blocks, Straggler<Real> *out_
     const cu_tile_desc &t = a.tiles[tile];
     const uint spp = t.spp, lane = cu::lane_id();
     uint bw, bh;
-    wave_pixel_block(spp, bw, bh);
+    wave_pixel_block(spp, bw, bh, a.wave_shape);
     const uint b = wave_global - a.block_offset[tile];
     const uint nbx = (t.width + bw - 1) / bw;
     const uint pix = lane / spp, s = lane % spp;

@@ -317,7 +317,8 @@ typedef struct cu_frame_tune {
     uint32_t block_steps;   /* 8 or 16 */
     uint32_t handoff_lanes; /* 1..32 */
     uint32_t tail_passes;   /* 1..8 */
-    uint32_t reserved;
+    uint32_t wave_shape;    /* 0..3: pass-1 wave covers a bw x bh pixel block, bw * bh = 64/spp;
+                               0 near-square (spp 1: 8x8), each step twice as wide (16x4, 32x2, 64x1) */
 } cu_frame_tune;
 typedef struct cu_frame_s *cu_frame;
 int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, int precision,
@@ -331,8 +332,16 @@ int cu_frame_outputs(cu_frame f, uint32_t **d_counts, double **d_smooth, uint64_
 /* Iterations executed by the last render, in all passes and in pass 1 alone
  * (pass1 may be NULL).  Synchronises the frame's stream. */
 int cu_frame_iterations(cu_frame f, unsigned long long *total, unsigned long long *pass1);
-/* Milliseconds of the last render's passes (pass 1 .. combine), HIP events. */
+/* Milliseconds of the last render: pass 1 and the whole render, HIP events on
+ * its stream (pass1_ms is 0 after a graph replay). */
 int cu_frame_last_ms(cu_frame f, float *pass1_ms, float *total_ms);
+/* Capture one render (same arguments as cu_frame_render) into a HIP graph owned
+ * by the frame; every launch in it is a kernel (clears included), so the graph
+ * can be replayed any number of times.  Recapturing replaces the graph. */
+int cu_frame_capture(cu_frame f, const cu_frame_tune *tune, const uint32_t *d_palette,
+                     uint32_t palette_length, uint32_t *d_rgba);
+/* One launch of the captured graph on `stream`. */
+int cu_frame_replay(cu_frame f, cu_stream stream);
 
 /* ---- deep zoom: perturbation around a double-double reference orbit ----
  * No reference counterpart: the reference stops at its double kernel
