@@ -10,7 +10,8 @@ value = Giga-iterations/s over the timed steps (iterations executed; an
 interior sample counts maxIter).
 
 Also reported (variants): the 256-tile uneven batch (autotuned, captured
-replay), the 1920x1080 deep-zoom frames (FP32 and FP64-delta perturbation and
+replay), 16 zoom sessions through the continuous-batching loop (p50/p99
+arrival-to-completion latency), the 1920x1080 deep-zoom frames (FP32 and FP64-delta perturbation and
 direct FP64 with their mismatch against the double-double oracle subset), and
 the roofline of the dominant kernel (pass 1 of the pipeline:
 8 algorithmic FP64 FLOPs per iteration it executes, against the 78.6 TFLOP/s
@@ -225,6 +226,23 @@ def deep_leg(cu, engine):
     return out
 
 
+def serving_leg(cu, hbm_budget=16 << 30, slots=8):
+    """16 zoom sessions, Poisson arrivals at 30 fps each, 120 frames, replayed in
+    real time through the continuous-batching loop."""
+    sv = importlib.import_module("chaos-ultra_amd.serve")
+    sessions = sv.zoom_sessions(16, 120)
+    server = sv.Server(cu.KERNELS_DIR, hbm_budget, slots=slots)
+    try:
+        recs, span = sv.run(server, sessions, realtime=True)
+        out = sv.summarise(recs, sessions, server.report(), span)
+    finally:
+        server.close()
+    out.update({"hbm_budget_bytes": hbm_budget, "slots": slots, "deep_method": "fp32 perturbation",
+                "start_widths": "log-uniform in [1e-14, 3]; frames below 1e-12 use perturbation",
+                "resolutions": [f"{s['width']}x{s['height']}" for s in sessions]})
+    return out
+
+
 def read_traffic():
     """HBM bytes per pass-1 launch from the committed counter summary, if present."""
     try:
@@ -321,6 +339,9 @@ def main():
         u = variants["uneven_tiles_256"]
         log(f"[bench] uneven tiles: pipeline {u['tuned_pipeline']}, winner {u['autotune']['engine']} "
             f"{u.get('tuned_persistent', u['tuned_pipeline'])}")
+        log("[bench] serving: 16 zoom sessions in real time")
+        variants["serving_16_sessions"] = serving_leg(cu)
+        log(f"[bench] serving: {variants['serving_16_sessions']}")
         log("[bench] deep zoom 1920x1080")
         variants["deep_zoom_1080p"] = deep_leg(cu, engine)
         log(f"[bench] deep zoom: {json.dumps(variants['deep_zoom_1080p'])}")

@@ -14,6 +14,7 @@ import numpy as np
 
 from . import abi
 from .abi import (  # noqa: F401  (re-exported)
+    ServeConfig, FrameRequest, FrameDone, ServeReport,
     KERNELS_DIR, LIB_PATH, ChaosError, CudaInitializationError, DeviceError, FractalRendererError,
     IllegalArgumentError, IllegalStateError, RenderingModel, TileDesc, Tune, FrameTune,
     DeepDesc, DeepReport, DEEP_DIRECT_F64, DEEP_PERTURB_F32, DEEP_PERTURB_F64, DEEP_REBASE,
@@ -435,6 +436,11 @@ class DeepFrame:
                rgba: DeviceBuffer | None = None, stream: Stream | None = None) -> None:
         abi.call("cu_deep_render", self.handle, palette.ptr if palette else None, palette_length,
                  rgba.ptr if rgba else None, stream.handle if stream else None)
+
+    def retarget(self, desc: DeepDesc) -> None:
+        """Another view of the same size (max_iter no larger than at creation)."""
+        abi.call("cu_deep_retarget", self.handle, C.byref(desc))
+        self.desc = desc
 
     def _download(self, which: int, dtype) -> np.ndarray:
         out = np.empty(self.pixels, dtype=dtype)

@@ -152,6 +152,36 @@ class DeepReport(C.Structure):
                 ("reserved2", C.c_float)]
 
 
+class ServeConfig(C.Structure):
+    """cu_serve_config"""
+    _fields_ = [("hbm_budget", C.c_uint64), ("deep_width", C.c_double), ("slots", C.c_uint32),
+                ("deep_method", C.c_uint32), ("max_super_sampling", C.c_float),
+                ("reserved", C.c_uint32)]
+
+
+class FrameRequest(C.Structure):
+    """cu_frame_request"""
+    _fields_ = [("centre_x", C.c_double), ("centre_y", C.c_double), ("width", C.c_double),
+                ("tag", C.c_uint64), ("max_iter", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class FrameDone(C.Structure):
+    """cu_frame_done"""
+    _fields_ = [("tag", C.c_uint64), ("session", C.c_uint32), ("path", C.c_uint32),
+                ("submit_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double),
+                ("iterations", C.c_ulonglong), ("checksum", C.c_ulonglong)]
+
+
+class ServeReport(C.Structure):
+    """cu_serve_report"""
+    _fields_ = [("iterations", C.c_ulonglong), ("frames", C.c_uint64),
+                ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64),
+                ("sessions_admitted", C.c_uint32), ("sessions_waiting", C.c_uint32)]
+
+
+assert C.sizeof(ServeConfig) == 32 and C.sizeof(FrameRequest) == 40
+assert C.sizeof(FrameDone) == 56 and C.sizeof(ServeReport) == 40
+
 DEEP_DIRECT_F64 = 1
 DEEP_PERTURB_F32 = 32
 DEEP_PERTURB_F64 = 64
@@ -235,10 +265,20 @@ SIGNATURES = {
     "cu_engine_configure": (_I, [_P, C.POINTER(Tune)]),
     "cu_deep_create": (_I, [_P, C.POINTER(DeepDesc), _I, C.POINTER(_P)]),
     "cu_deep_destroy": (_I, [_P]),
+    "cu_deep_retarget": (_I, [_P, C.POINTER(DeepDesc)]),
     "cu_deep_render": (_I, [_P, _P, _U32, _P, _P]),
     "cu_deep_outputs": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "cu_deep_stats": (_I, [_P, C.POINTER(DeepReport)]),
     "cu_deep_orbit": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
+    "cu_serve_create": (_I, [C.c_char_p, C.POINTER(ServeConfig), C.POINTER(_P)]),
+    "cu_serve_destroy": (_I, [_P]),
+    "cu_serve_open": (_I, [_P, _U32, _U32, _P, _U32, _U32, C.POINTER(_U32)]),
+    "cu_serve_submit": (_I, [_P, _U32, C.POINTER(FrameRequest)]),
+    "cu_serve_close": (_I, [_P, _U32]),
+    "cu_serve_poll": (_I, [_P, C.POINTER(FrameDone), _U32, C.POINTER(_U32), _I]),
+    "cu_serve_read_rgba": (_I, [_P, _U32, _P, _SZ]),
+    "cu_serve_stats": (_I, [_P, C.POINTER(ServeReport)]),
+    "cu_serve_session_times": (_I, [_P, _U32, C.POINTER(_D), C.POINTER(_D)]),
 }
 
 _lib = None

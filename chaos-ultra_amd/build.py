@@ -39,7 +39,7 @@ HOST_FLAGS = [
     "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off",
     "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
 ]
-HOST_SOURCES = ("driver.cpp", "renderer.cpp", "engine.cpp", "frame.cpp", "deep.cpp")
+HOST_SOURCES = ("driver.cpp", "renderer.cpp", "engine.cpp", "frame.cpp", "deep.cpp", "serve.cpp")
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:

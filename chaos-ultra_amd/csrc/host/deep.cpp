@@ -64,6 +64,7 @@ void dd_add_double(double a_hi, double a_lo, double b, double &hi, double &lo) {
 struct cu_deep_s {
     cu_engine engine = nullptr;
     cu_deep_desc desc{};
+    uint32_t capacity_iter = 0;  // max_iter the orbit buffers were sized for
     int method = CU_DEEP_PERTURB_F64;
     bool fused = false;  // direct form: fused step variant
     size_t entry_bytes = 32;
@@ -210,6 +211,7 @@ int cu_deep_create(cu_engine e, const cu_deep_desc *desc, int method, cu_deep *o
         d = new cu_deep_s();
         d->engine = e;
         d->desc = *desc;
+        d->capacity_iter = desc->max_iter;
         d->method = method;
         d->entry_bytes = method == CU_DEEP_PERTURB_F32 ? 16 : 32;
         if (method == CU_DEEP_DIRECT_F64) {
@@ -240,6 +242,23 @@ int cu_deep_create(cu_engine e, const cu_deep_desc *desc, int method, cu_deep *o
     });
     if (rc != CU_OK && d) free_deep(d);
     return rc;
+}
+
+int cu_deep_retarget(cu_deep d, const cu_deep_desc *desc) {
+    return guarded([&] {
+        if (!d || !desc) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        validate_deep(*desc, d->method);
+        if (desc->width != d->desc.width || desc->height != d->desc.height)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "retarget keeps the frame size");
+        if (desc->max_iter > d->capacity_iter)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "max_iter above the capacity the frame was created with");
+        const bool rounds_before = !(d->desc.flags & CU_DEEP_REBASE) && d->desc.max_references > 0;
+        const bool rounds_after = !(desc->flags & CU_DEEP_REBASE) && desc->max_references > 0;
+        if (rounds_after && !rounds_before)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "glitch rounds need a frame created without rebasing");
+        std::lock_guard<std::mutex> lock(d->engine->mutex);
+        d->desc = *desc;
+    });
 }
 
 int cu_deep_destroy(cu_deep d) {

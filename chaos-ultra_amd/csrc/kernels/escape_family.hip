@@ -371,6 +371,18 @@ extern "C" __global__ __launch_bounds__(256) void composeSmooth(
         }
 }
 
+// Order-sensitive checksum of n words: sum of p[i] * (2i + 1) mod 2^64, added to
+// *out (integer addition, so the result does not depend on the schedule).
+extern "C" __global__ __launch_bounds__(256) void checksumU32(const uint *p, uint64_t n,
+                                                              unsigned long long *out) {
+    uint64_t acc = 0;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (uint64_t)gridDim.x * blockDim.x)
+        acc += (uint64_t)p[k] * (2ull * k + 1ull);
+    acc = wave_sum(acc);
+    if (cu::lane_id() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
 extern "C" __global__ __launch_bounds__(256) void fillU32(uint *p, uint value, uint64_t n) {
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (uint64_t)gridDim.x * blockDim.x)

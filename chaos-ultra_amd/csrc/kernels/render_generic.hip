@@ -31,6 +31,12 @@
 
 __constant__ bool VISUALIZE_SAMPLE_COUNT = false;
 
+// Iterations executed by this module's render kernels, CU_ITER_SLOTS u64 slots
+// (each wave adds its total to slot `wave mod CU_ITER_SLOTS`).  A host that wants
+// totals zeroes the symbol, renders and sums it; kernel parameter lists are
+// unchanged.  Not part of the reference's module ABI.
+__device__ unsigned long long cu_iteration_slots[CU_ITER_SLOTS];
+
 namespace {
 
 constexpr uint kBlockSteps = 16;       // unchecked iterations per block
@@ -84,12 +90,25 @@ __device__ __forceinline__ pixel_info_t load_pixel(const pixel_info_t *src) {
 
 // computeFractal<Real> of the module, bailout 4 (the reference's constant).
 template <class Real>
-__device__ __forceinline__ float compute_fractal(uint max_iter, Real X, Real Y, bool blocks) {
+__device__ __forceinline__ float compute_fractal(uint max_iter, Real X, Real Y, bool blocks,
+                                                 uint64_t &executed) {
     Real zx, zy, cx, cy;
     Fractal::start<Real>(X, Y, zx, zy, cx, cy);
     const cu::Orbit<Real> o =
         cu::escape_orbit<Real, kBlockSteps, false>(zx, zy, cx, cy, max_iter, (Real)4, blocks);
+    executed += o.n;
     return Fractal::value(o.n, max_iter);
+}
+
+// Add a wave's executed iterations to cu_iteration_slots (whole wave calls it).
+__device__ __forceinline__ void count_iterations(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    const uint tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const uint waves = (blockDim.x * blockDim.y + 63u) >> 6;
+    const uint wave = (blockIdx.y * gridDim.x + blockIdx.x) * waves + (tid >> 6);
+    if (cu::lane_id() == 0 && v)
+        atomicAdd(cu_iteration_slots + (wave & (CU_ITER_SLOTS - 1u)), (unsigned long long)v);
 }
 
 // computeDispersion (fractalRendererGeneric.cu:24-34) over the stored samples;
@@ -110,7 +129,7 @@ __device__ __forceinline__ float dispersion(const float (&samples)[kAdaptiveThre
 template <class Real>
 __device__ uint sample_the_fractal(int site, Point<uint> pixel, Point<uint> grid,
                                    Rectangle<Real> image, uint max_iter, float &sample_count_f,
-                                   bool adaptive) {
+                                   bool adaptive, uint64_t &executed) {
     const uint64_t half = (cu::lane_id() < 32) ? 0x00000000ffffffffull : 0xffffffff00000000ull;
     const uint64_t site1 = cu::ballot(site == 1), site2 = cu::ballot(site == 2);
     const uint64_t group = half & (site == 1 ? site1 : (site == 2 ? site2 : 0ull));
@@ -147,7 +166,7 @@ __device__ uint sample_the_fractal(int site, Point<uint> pixel, Point<uint> grid
             }
             const Real X = image.left_bottom.x + ((Real)pixel.x + dx) * psx;
             const Real Y = image.right_top.y - ((Real)pixel.y + dy) * psy;
-            const uint escape_time = (uint)compute_fractal<Real>(max_iter, X, Y, blocks);
+            const uint escape_time = (uint)compute_fractal<Real>(max_iter, X, Y, blocks, executed);
             escape_sum += escape_time;
 #pragma unroll
             for (uint j = 0; j < kAdaptiveThreshold; ++j)
@@ -189,8 +208,10 @@ __device__ void render_main(pixel_info_t *out, long out_pitch, Point<uint> size,
     const Point<uint> idx = image_indexes();
     const bool inside = idx.x < size.x && idx.y < size.y;
     float sample_count = max_ss;
+    uint64_t executed = 0;
     const uint r = sample_the_fractal<Real>(inside ? 1 : 0, idx, size, image, max_iter,
-                                            sample_count, (flags & kAdaptiveFlag) != 0);
+                                            sample_count, (flags & kAdaptiveFlag) != 0, executed);
+    count_iterations(executed);
     if (inside) {
         pixel_info_t p;
         p.value = (float)r;
@@ -315,8 +336,10 @@ __device__ void render_advanced(pixel_info_t *out, long out_pitch, Point<uint> s
             site = 2;
         }
     }
+    uint64_t executed = 0;
     const uint r = sample_the_fractal<Real>(site, idx, size, image, max_iter, sample_count,
-                                            (flags & kAdaptiveFlag) != 0);
+                                            (flags & kAdaptiveFlag) != 0, executed);
+    count_iterations(executed);
     if (site == 1) {  // :349-356
         const float samples = (float)r;
         reused.weight = (float)((double)reused.weight * 0.75);
@@ -430,7 +453,9 @@ extern "C" __global__ void fractalRenderUnderSampled(pixel_info_t *output, long 
     const float ph = (right_top_y - left_bottom_y) / (float)height;
     const float cx = left_bottom_x + idx_x * pw;
     const float cy = right_top_y - idx_y * ph;
-    const uint e = (uint)compute_fractal<float>(maxIterations, cx, cy, Fractal::blocks_valid(4.0));
+    uint64_t executed = 0;
+    const uint e =
+        (uint)compute_fractal<float>(maxIterations, cx, cy, Fractal::blocks_valid(4.0), executed);
     pixel_info_t p;
     p.value = (float)e;
     p.weight = 1 / (float)underSamplingLevel;

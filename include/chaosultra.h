@@ -388,6 +388,10 @@ typedef struct cu_deep_report {
 typedef struct cu_deep_s *cu_deep;
 int cu_deep_create(cu_engine e, const cu_deep_desc *d, int method, cu_deep *out);
 int cu_deep_destroy(cu_deep d);
+/* Point a deep frame at another view of the same size and method (centre, pixel
+ * sizes, bailout, flags; max_iter no larger than at creation).  Device buffers
+ * are kept. */
+int cu_deep_retarget(cu_deep d, const cu_deep_desc *desc);
 /* d_palette / d_rgba may be NULL.  Synchronous when glitch rounds run. */
 int cu_deep_render(cu_deep d, const uint32_t *d_palette, uint32_t palette_length,
                    uint32_t *d_rgba, cu_stream stream);
@@ -398,6 +402,67 @@ int cu_deep_stats(cu_deep d, cu_deep_report *out);
 /* Copy of the primary reference orbit of the last render: entries of 4 values
  * {Zx, Zy, |Z|^2/2, 0} in the delta precision; *length = M (entries M + 1). */
 int cu_deep_orbit(cu_deep d, void *host_out, uint32_t capacity_entries, uint32_t *length);
+
+/* ---- serving: continuous batching of zoom-session frame requests ----
+ * Many sessions, one GPU.  A session owns the reference's ping-pong
+ * pixel_info_t buffers, an RGBA8 image, its palette and a deep-zoom frame, all
+ * resident so each frame reuses the previous frame's samples.  Sessions are
+ * admitted against hbm_budget (footprint reserved at admission, released at
+ * close; first come, first served).  Requests queue per session; whenever one
+ * of `slots` streams is free the oldest request of an idle admitted session
+ * starts on it, so sessions overlap on the device while each session's frames
+ * run in order.  A frame with width >= deep_width runs the mandelbrot module's
+ * fractalRenderAdvanced kernel (reuse + foveation around the view centre,
+ * zooming in; fractalRenderMain for a first frame) and compose
+ * (cudarenderer/CudaFractalRenderer.java:158-206); a narrower one runs the
+ * deep-zoom form (bailout 4, rebasing).  Replaces the single-session display
+ * loop of rendering/GLRenderer.java:113-162. */
+typedef struct cu_serve_config {
+    uint64_t hbm_budget;       /* bytes for admitted sessions */
+    double deep_width;         /* frames narrower than this use perturbation (e.g. 1e-12) */
+    uint32_t slots;            /* frames in flight, one HIP stream each: 1..16 */
+    uint32_t deep_method;      /* CU_DEEP_PERTURB_F32 or CU_DEEP_PERTURB_F64 */
+    float max_super_sampling;  /* maxSuperSampling of fresh pixels, 1..64 */
+    uint32_t reserved;
+} cu_serve_config;
+typedef struct cu_frame_request {
+    double centre_x, centre_y; /* view centre */
+    double width;              /* real-axis width of the view; height = width * H / W */
+    uint64_t tag;              /* returned with the completion */
+    uint32_t max_iter;
+    uint32_t reserved;
+} cu_frame_request;
+typedef struct cu_frame_done {
+    uint64_t tag;
+    uint32_t session;
+    uint32_t path;             /* 0 reuse (advanced kernel), 1 full (main kernel), 2 deep */
+    double submit_s, start_s, done_s; /* seconds since cu_serve_create */
+    unsigned long long iterations;    /* deep frames (module-kernel frames count in the total) */
+    unsigned long long checksum;      /* sum of rgba[i] * (2i + 1) mod 2^64 of the frame */
+} cu_frame_done;
+typedef struct cu_serve_report {
+    unsigned long long iterations;    /* every frame so far */
+    uint64_t frames;
+    uint64_t bytes_in_use, peak_bytes;
+    uint32_t sessions_admitted, sessions_waiting;
+} cu_serve_report;
+typedef struct cu_serve_s *cu_serve;
+int cu_serve_create(const char *kernels_dir, const cu_serve_config *cfg, cu_serve *out);
+int cu_serve_destroy(cu_serve s);
+/* deep_max_iter: largest max_iter of the session's deep frames (0: no deep frames). */
+int cu_serve_open(cu_serve s, uint32_t width, uint32_t height, const uint32_t *palette,
+                  uint32_t palette_length, uint32_t deep_max_iter, uint32_t *session);
+int cu_serve_submit(cu_serve s, uint32_t session, const cu_frame_request *req);
+/* Release the session's memory once its queued frames are done. */
+int cu_serve_close(cu_serve s, uint32_t session);
+/* Progress: retire finished frames into out[], admit waiting sessions, start
+ * ready frames.  Waits up to wait_ms for a completion while frames are in flight. */
+int cu_serve_poll(cu_serve s, cu_frame_done *out, uint32_t capacity, uint32_t *count,
+                  int wait_ms);
+/* The session's last completed image (no frame may be in flight). */
+int cu_serve_read_rgba(cu_serve s, uint32_t session, uint32_t *dst, size_t count);
+int cu_serve_stats(cu_serve s, cu_serve_report *out);
+int cu_serve_session_times(cu_serve s, uint32_t session, double *open_s, double *admit_s);
 
 #ifdef __cplusplus
 }

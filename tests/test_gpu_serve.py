@@ -1,0 +1,170 @@
+"""The serving loop (cu_serve_*) on the GPU.
+
+* a session's frames are the reference renderer's frames: the first is
+  fractalRenderMain + compose, the next ones fractalRenderAdvanced with reuse
+  and foveation around the centre + compose, each equal to the oracle chain
+  (render_main -> render_advanced ... -> compose); a deep frame equals the
+  oracle's FP32 perturbation + smooth compose;
+* every session's frames are the same alone and in the mix (per-frame image
+  checksums), with several frames in flight at once;
+* a small HBM budget admits sessions one at a time; all frames still complete,
+  with the same results, and the peak reservation stays within the budget.
+"""
+import importlib
+import math
+
+import numpy as np
+import pytest
+
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+sv = importlib.import_module("chaos-ultra_amd.serve")
+
+
+def rect(centre, width, W, H):
+    h = width * H / W
+    return (centre[0] - 0.5 * width, centre[1] - 0.5 * h, centre[0] + 0.5 * width, centre[1] + 0.5 * h)
+
+
+def wait_frame(server, n=1):
+    got = []
+    for _ in range(20000):
+        got += server.poll(wait_ms=5)
+        if len(got) >= n:
+            return got
+    raise TimeoutError("frame did not complete")
+
+
+def small_sessions(cu, n=4, frames=10, size=(192, 108)):
+    ss = sv.zoom_sessions(n, frames, seed=99, resolutions=(size,))
+    # one deep session and one that crosses into deep frames
+    ss[0]["widths"] = 5e-13 * sv.ZOOM ** np.arange(frames)
+    ss[1]["widths"] = 1.3e-12 * sv.ZOOM ** np.arange(frames)
+    for s in ss:
+        s["max_iter"] = [min(1500, sv.max_iter_for(w)) for w in s["widths"]]
+        s["arrivals"] = np.arange(frames) * 1e-3
+    return ss
+
+
+def session_bytes(s):
+    pitch = (s["width"] * 16 + 255) // 256 * 256
+    b = 2 * pitch * s["height"] + 4 * s["width"] * s["height"] + 256 * 4
+    deep = [m for m, w in zip(s["max_iter"], s["widths"]) if w < sv.DEEP_WIDTH]
+    if deep:
+        b += 13 * s["width"] * s["height"] + (max(deep) + 1) * 32 + 1024 * 8 + 8
+    return b
+
+
+def test_frames_match_oracle_chain(gpu):
+    cu = gpu
+    W, H, max_ss = 160, 90, 2.0
+    server = sv.Server(cu.KERNELS_DIR, 1 << 30, slots=2, max_super_sampling=max_ss)
+    pal = sv.palette256()
+    try:
+        sid = server.open(W, H, pal, 0)
+        centre = (-0.7436438870, 0.1318259042)
+        widths = [0.02 * 0.97 ** k for k in range(3)]
+        prev = None
+        for k, w in enumerate(widths):
+            seg = rect(centre, w, W, H)
+            server.submit(sid, centre, w, 600, k)
+            done = wait_frame(server)[0]
+            got = server.read_rgba(sid, W, H)
+            precision = 64 if cu.precision_for_segment(*seg, W, H) == cu.PRECISION_DOUBLE else 32
+            if k == 0:
+                assert done.path == 1
+                pix = O.render_main(precision, 0, (0.0, 0.0), W, H, seg, 600, max_ss, 1)
+            else:
+                assert done.path == 0
+                pix = O.render_advanced(precision, 0, (0.0, 0.0), W, H, seg, 600, max_ss,
+                                        1 | 4 | 8 | 16 | 32, prev_seg, prev, (W // 2, H // 2))
+            want = O.compose(pix, pal, max_ss, False)
+            assert np.array_equal(got, want), f"frame {k}"
+            prev, prev_seg = pix, seg
+        # a deep frame: the oracle's FP32 perturbation (bailout 4, rebasing) + smooth compose
+        dsid = server.open(96, 54, pal, 3000)
+        w = 4e-13
+        server.submit(dsid, centre, w, 3000, 99)
+        done = wait_frame(server)[0]
+        assert done.path == 2 and done.iterations > 0
+        got = server.read_rgba(dsid, 96, 54)
+        d = cu.make_deep(centre[0], centre[1], w, 96, 54, 3000, 4.0)
+        orbit, M = O.reference_orbit(centre[0], centre[1], 3000, 4.0, 32)
+        ys, xs = np.mgrid[0:54, 0:96]
+        c, mag, st, _ = O.deep_pixels(O.DEEP_PERTURB_F32, xs.ravel(), ys.ravel(), d, orbit, M)
+        sm = np.where(st == 1, c + 1.0 - np.log2(np.log(np.where(st == 1, mag, 4.0)) / math.log(4.0)), 0.0)
+        assert np.array_equal(got, O.compose_smooth(sm.reshape(54, 96), pal))
+    finally:
+        server.close()
+
+
+def run_alone_and_mixed(cu, sessions, budget, slots=4):
+    alone = {}
+    for s in sessions:
+        server = sv.Server(cu.KERNELS_DIR, 1 << 32, slots=slots)
+        try:
+            recs, _ = sv.run(server, [s], realtime=False)
+            alone[s["id"]] = [r["checksum"] for r in recs]
+            assert [r["frame"] for r in recs] == list(range(len(s["widths"])))
+        finally:
+            server.close()
+    server = sv.Server(cu.KERNELS_DIR, budget, slots=slots)
+    try:
+        recs, _ = sv.run(server, sessions, realtime=False)
+        rep = server.report()
+        mixed = {}
+        for r in recs:
+            mixed.setdefault(sessions[r["session"]]["id"], []).append(r["checksum"])
+        return alone, mixed, rep, recs
+    finally:
+        server.close()
+
+
+def test_same_alone_and_in_the_mix(gpu):
+    cu = gpu
+    sessions = small_sessions(cu)
+    alone, mixed, rep, recs = run_alone_and_mixed(cu, sessions, 1 << 32)
+    assert alone == mixed
+    paths = {r["path"] for r in recs}
+    assert paths == {0, 1, 2}
+    assert rep.frames == sum(len(s["widths"]) for s in sessions)
+    assert rep.iterations > 0
+    # frames of different sessions were in flight together
+    spans = sorted((r["submit_s"] + r["queue_s"], r["done_s"]) for r in recs)
+    overlap = any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    assert overlap
+
+
+def test_admission_by_hbm_budget(gpu):
+    cu = gpu
+    sessions = small_sessions(cu)
+    need = [session_bytes(s) for s in sessions]
+    budget = int(max(need) * 1.5)  # one session at a time
+    alone, mixed, rep, recs = run_alone_and_mixed(cu, sessions, budget)
+    assert alone == mixed
+    assert rep.peak_bytes <= budget
+    assert rep.sessions_admitted == len(sessions)
+    # sessions waited for admission: their first frame started after an earlier session closed
+    firsts = sorted(r["submit_s"] + r["queue_s"] for r in recs if r["frame"] == 0)
+    lasts = sorted(r["done_s"] for r in recs if r["frame"] == len(sessions[0]["widths"]) - 1)
+    assert firsts[-1] >= lasts[0]
+
+
+def test_serve_rejects(gpu):
+    cu = gpu
+    with pytest.raises(cu.IllegalArgumentError, match="slots"):
+        sv.Server(cu.KERNELS_DIR, 1 << 30, slots=0)
+    server = sv.Server(cu.KERNELS_DIR, 1 << 20, slots=1)
+    try:
+        with pytest.raises(cu.IllegalArgumentError, match="budget"):
+            server.open(3840, 2160, sv.palette256(), 0)
+        sid = server.open(64, 36, sv.palette256(), 0)
+        with pytest.raises(cu.IllegalArgumentError, match="deep_max_iter"):
+            server.submit(sid, (-0.75, 0.1), 1e-13, 1000, 0)
+        server.close_session(sid)
+        with pytest.raises(cu.IllegalStateError):
+            server.submit(sid, (-0.75, 0.1), 1e-3, 1000, 0)
+    finally:
+        server.close()
