@@ -219,6 +219,15 @@ def deep_leg(cu, engine):
                          "mismatch_vs_dd_pct": round(100 * float(np.mean(counts != g[f"counts_{w:.0e}"])), 3)}
             if method != cu.DEEP_DIRECT_F64:
                 row[name]["orbit_ms"] = round(rep.orbit_ms, 3)
+                # a zoom sequence toward the same centre keeps the orbit between frames
+                d.flags = cu.DEEP_REBASE | cu.DEEP_REUSE_ORBIT
+                f = cu.DeepFrame(engine, d, method)
+                kept = []
+                for _ in range(3):
+                    f.render()
+                    kept.append(f.report().total_ms)
+                f.close()
+                row[name]["frame_ms_orbit_kept"] = round(float(np.median(kept[1:])), 3)
         if "direct_f64" in row:
             row["perturb_f32_over_direct_f64"] = round(row["perturb_f32"]["giga_iter_s"] /
                                                        row["direct_f64"]["giga_iter_s"], 3)

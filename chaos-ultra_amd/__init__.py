@@ -18,6 +18,7 @@ from .abi import (  # noqa: F401  (re-exported)
     KERNELS_DIR, LIB_PATH, ChaosError, CudaInitializationError, DeviceError, FractalRendererError,
     IllegalArgumentError, IllegalStateError, RenderingModel, TileDesc, Tune, FrameTune,
     DeepDesc, DeepReport, DEEP_DIRECT_F64, DEEP_PERTURB_F32, DEEP_PERTURB_F64, DEEP_REBASE,
+    DEEP_REUSE_ORBIT,
     FLAG_ADAPTIVE_SS, FLAG_FOVEATION, FLAG_SAMPLE_REUSE, FLAG_VISUALISE_SAMPLE_COUNT,
     FLAG_ZOOMING, FLAG_ZOOMING_IN, FRACTAL_JULIA, FRACTAL_MANDELBROT, PRECISION_DOUBLE,
     PRECISION_SINGLE, PRECISION_TOO_BIG, SAMPLE_CENTRE, SAMPLE_CORNER, SAMPLE_JITTER,

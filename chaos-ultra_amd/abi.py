@@ -186,6 +186,7 @@ DEEP_DIRECT_F64 = 1
 DEEP_PERTURB_F32 = 32
 DEEP_PERTURB_F64 = 64
 DEEP_REBASE = 1
+DEEP_REUSE_ORBIT = 2
 
 # name -> (restype, argtypes)
 _P = C.c_void_p

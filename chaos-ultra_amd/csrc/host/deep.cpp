@@ -65,6 +65,10 @@ struct cu_deep_s {
     cu_engine engine = nullptr;
     cu_deep_desc desc{};
     uint32_t capacity_iter = 0;  // max_iter the orbit buffers were sized for
+    // primary orbit held in d_orbit[0]: its centre, bailout and iteration limit
+    bool orbit_valid = false;
+    double orbit_key[5] = {0, 0, 0, 0, 0};
+    uint32_t orbit_mi = 0;
     int method = CU_DEEP_PERTURB_F64;
     bool fused = false;  // direct form: fused step variant
     size_t entry_bytes = 32;
@@ -112,7 +116,8 @@ void validate_deep(const cu_deep_desc &d, int method) {
         fail(CU_ERR_ILLEGAL_ARGUMENT, "max_references must be <= 16");
     if (!(d.flags & CU_DEEP_REBASE) && !(d.glitch_eps > 0.0f && d.glitch_eps < 1.0f))
         fail(CU_ERR_ILLEGAL_ARGUMENT, "glitch_eps must be in (0, 1) without rebasing");
-    if (d.flags & ~CU_DEEP_REBASE) fail(CU_ERR_ILLEGAL_ARGUMENT, "unknown flags");
+    if (d.flags & ~(CU_DEEP_REBASE | CU_DEEP_REUSE_ORBIT))
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "unknown flags");
 }
 
 const char *orbit_kernel(const cu_deep_s *d) {
@@ -120,9 +125,8 @@ const char *orbit_kernel(const cu_deep_s *d) {
 }
 
 void launch_orbit(cu_deep d, cu_stream s, int slot, double cx_hi, double cx_lo, double cy_hi,
-                  double cy_lo) {
+                  double cy_lo, uint32_t mi) {
     cu_function f = function(d->engine, orbit_kernel(d));
-    uint32_t mi = d->desc.max_iter;
     double bsq = d->desc.bailout_sq;
     void *orbit = d->d_orbit[slot];
     uint32_t *len = (uint32_t *)d->d_len + slot;
@@ -163,7 +167,8 @@ void launch_pixels(cu_deep d, cu_stream s, DeepArgs &a) {
     else
         name = d->method == CU_DEEP_PERTURB_F32 ? "perturbFloat" : "perturbDouble";
     cu_function f = function(d->engine, name);
-    void *params[] = {&a};
+    const void *orbit = a.orbit;
+    void *params[] = {&a, &orbit};  // direct kernels take only the first
     dim3 grid((d->desc.width + 15) / 16, (d->desc.height + 15) / 16, 1);
     launch(d->engine, f, grid, dim3(256), params, s, false);
 }
@@ -280,14 +285,29 @@ int cu_deep_render(cu_deep d, const uint32_t *d_palette, uint32_t palette_length
         ensure_device();
         hipStream_t st = stream ? stream->stream : nullptr;
         const cu_deep_desc &c = d->desc;
+        // The primary orbit is kept when the caller allows it and the centre, bailout
+        // and a large enough iteration limit match: an orbit computed to more
+        // iterations than a frame uses gives that frame the same result (its pixels
+        // stop at max_iter before reaching the orbit's end).
+        const double key[5] = {c.centre_x, c.centre_x_lo, c.centre_y, c.centre_y_lo, c.bailout_sq};
+        const bool keep = (c.flags & CU_DEEP_REUSE_ORBIT) && d->orbit_valid &&
+                          std::memcmp(key, d->orbit_key, sizeof(key)) == 0 &&
+                          c.max_iter <= d->orbit_mi;
         fill_u32(d->engine, stream, d->d_iters, 0, CU_ITER_SLOTS * 2);
-        fill_u32(d->engine, stream, d->d_len, 0, 2);
+        if (keep)
+            fill_u32(d->engine, stream, (uint32_t *)d->d_len + 1, 0, 1);
+        else
+            fill_u32(d->engine, stream, d->d_len, 0, 2);
         check(hipEventRecord(d->ev[0], st), "hipEventRecord");
         d->references = 0;
         d->glitched = 0;
-        if (d->method != CU_DEEP_DIRECT_F64) {
-            launch_orbit(d, stream, 0, c.centre_x, c.centre_x_lo, c.centre_y, c.centre_y_lo);
+        if (d->method != CU_DEEP_DIRECT_F64 && !keep) {
+            const uint32_t mi = (c.flags & CU_DEEP_REUSE_ORBIT) ? d->capacity_iter : c.max_iter;
+            launch_orbit(d, stream, 0, c.centre_x, c.centre_x_lo, c.centre_y, c.centre_y_lo, mi);
             d->references = 1;
+            d->orbit_valid = true;
+            std::memcpy(d->orbit_key, key, sizeof(key));
+            d->orbit_mi = mi;
         }
         check(hipEventRecord(d->ev[1], st), "hipEventRecord");
         DeepArgs a = base_args(d, d_palette, palette_length, d_rgba);
@@ -310,7 +330,7 @@ int cu_deep_render(cu_deep d, const uint32_t *d_palette, uint32_t palette_length
                 double rx_hi, rx_lo, ry_hi, ry_lo;
                 dd_add_double(c.centre_x, c.centre_x_lo, ox, rx_hi, rx_lo);
                 dd_add_double(c.centre_y, c.centre_y_lo, oy, ry_hi, ry_lo);
-                launch_orbit(d, stream, 1, rx_hi, rx_lo, ry_hi, ry_lo);
+                launch_orbit(d, stream, 1, rx_hi, rx_lo, ry_hi, ry_lo, c.max_iter);
                 ++d->references;
                 DeepArgs g = a;
                 g.orbit = d->d_orbit[1];

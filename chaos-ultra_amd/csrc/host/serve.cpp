@@ -149,7 +149,7 @@ void admit(cu_serve_s *s) {
         if (ss.deep_max_iter) {
             const double pw = 1e-13 / ss.width;
             cu_deep_desc d{-0.75, 0.0, 0, 0, pw, pw, 4.0, ss.width, ss.height, ss.deep_max_iter,
-                           CU_DEEP_REBASE, 0, 1e-6f};
+                           CU_DEEP_REBASE | CU_DEEP_REUSE_ORBIT, 0, 1e-6f};
             if (cu_deep_create(s->engine, &d, (int)s->cfg.deep_method, &ss.deep) != CU_OK)
                 fail(CU_ERR_DEVICE, cu_last_error());
         }
@@ -181,7 +181,7 @@ void dispatch(cu_serve_s *s, Session &ss, Slot &sl, const Request &rq) {
             fail(CU_ERR_ILLEGAL_ARGUMENT, "max_iter above the session's deep_max_iter");
         const double pw = r.width / W;
         cu_deep_desc d{r.centre_x, r.centre_y, 0, 0, pw, pw, 4.0, W, H, r.max_iter,
-                       CU_DEEP_REBASE, 0, 1e-6f};
+                       CU_DEEP_REBASE | CU_DEEP_REUSE_ORBIT, 0, 1e-6f};
         if (cu_deep_retarget(ss.deep, &d) != CU_OK) fail(CU_ERR_ILLEGAL_ARGUMENT, cu_last_error());
         if (cu_deep_render(ss.deep, (const uint32_t *)ss.palette, (uint32_t)ss.host_palette.size(),
                            (uint32_t *)ss.rgba, &sl.wrap) != CU_OK)

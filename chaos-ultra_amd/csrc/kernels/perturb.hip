@@ -172,7 +172,31 @@ __device__ __forceinline__ int checked_step(const OrbitEntry<Real> *Z, uint M, R
     return 0;
 }
 
+// K unchecked steps from orbit entry zp[0]; returns the ballot mask of lanes
+// whose rebase test fired at any of the K states.
+template <class Real, int K>
+__device__ __forceinline__ uint64_t unchecked_block(const OrbitEntry<Real> *zp, Real &dx, Real &dy,
+                                                    Real dcx, Real dcy) {
+    uint64_t flagged = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const OrbitEntry<Real> e = zp[k];
+        const Real t = cu::fma_r(e.x, dx, cu::fma_r(e.y, dy, e.h));
+        flagged |= cu::ballot(t < (Real)0);
+        const Real ax = cu::fma_r((Real)2, e.x, dx);
+        const Real ay = cu::fma_r((Real)2, e.y, dy);
+        const Real ndx = cu::fma_r(ax, dx, cu::fma_r(-ay, dy, dcx));
+        const Real ndy = cu::fma_r(ax, dy, cu::fma_r(ay, dx, dcy));
+        dx = ndx;
+        dy = ndy;
+    }
+    return flagged;
+}
+
 // Whole pixel; lanes whose `active` is false only take part in the ballots.
+// When every lane running a block reads the same orbit entries (same m), the
+// block reads them once per wave through the scalar cache instead of once per
+// lane through the vector memory path.
 template <class Real, int K>
 __device__ int perturb_pixel(const OrbitEntry<Real> *Z, uint M, Real dcx, Real dcy, uint max_iter,
                              Real bsq, bool rebase, Real geps, bool active, uint &n_out,
@@ -189,20 +213,10 @@ __device__ int perturb_pixel(const OrbitEntry<Real> *Z, uint M, Real dcx, Real d
         if (blk) {
             const Real sx = dx, sy = dy;
             const uint sm = m, sn = n;
-            const OrbitEntry<Real> *zp = Z + m;
-            uint64_t flagged = 0;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const OrbitEntry<Real> e = zp[k];
-                const Real t = cu::fma_r(e.x, dx, cu::fma_r(e.y, dy, e.h));
-                flagged |= cu::ballot(t < (Real)0);
-                const Real ax = cu::fma_r((Real)2, e.x, dx);
-                const Real ay = cu::fma_r((Real)2, e.y, dy);
-                const Real ndx = cu::fma_r(ax, dx, cu::fma_r(-ay, dy, dcx));
-                const Real ndy = cu::fma_r(ax, dy, cu::fma_r(ay, dx, dcy));
-                dx = ndx;
-                dy = ndy;
-            }
+            const uint mu = __builtin_amdgcn_readfirstlane(m);
+            const uint64_t flagged = cu::ballot(m != mu) == 0
+                                         ? unchecked_block<Real, K>(Z + mu, dx, dy, dcx, dcy)
+                                         : unchecked_block<Real, K>(Z + m, dx, dy, dcx, dcy);
             m += K;
             n += K;
             const OrbitEntry<Real> e = Z[m];
@@ -249,7 +263,7 @@ __device__ __forceinline__ void deep_pixel_coords(uint &px, uint &py) {
 }
 
 template <class Real, int K>
-__device__ void perturb_frame(const DeepArgs a) {
+__device__ void perturb_frame(const DeepArgs a, const OrbitEntry<Real> *__restrict__ orbit) {
     uint px, py;
     deep_pixel_coords(px, py);
     const uint64_t o = (uint64_t)py * a.width + px;
@@ -266,7 +280,7 @@ __device__ void perturb_frame(const DeepArgs a) {
     uint n = 0;
     Real mag = 0;
     const int state = perturb_pixel<Real, K>(
-        (const OrbitEntry<Real> *)a.orbit, M, dcx, dcy, a.max_iter, (Real)a.bailout_sq,
+        orbit, M, dcx, dcy, a.max_iter, (Real)a.bailout_sq,
         (a.flags & kDeepRebase) != 0, (Real)a.glitch_eps, active, n, mag);
     if (active) deep_write(a, o, state, n, (double)mag);
     add_iterations(a.iters, active ? (uint64_t)n : 0u);
@@ -303,11 +317,15 @@ extern "C" __global__ __launch_bounds__(64) void referenceOrbitDouble(
     OrbitEntry<double> *orbit, uint *length) {
     reference_orbit<double>(cx_hi, cx_lo, cy_hi, cy_lo, max_iter, bailout_sq, orbit, length);
 }
-extern "C" __global__ __launch_bounds__(256) void perturbFloat(DeepArgs a) {
-    perturb_frame<float, 8>(a);
+// `orbit` repeats a.orbit as its own read-only argument, so uniform loads from
+// it can go through the scalar cache.
+extern "C" __global__ __launch_bounds__(256) void perturbFloat(
+    DeepArgs a, const OrbitEntry<float> *__restrict__ orbit) {
+    perturb_frame<float, 8>(a, orbit);
 }
-extern "C" __global__ __launch_bounds__(256) void perturbDouble(DeepArgs a) {
-    perturb_frame<double, 8>(a);
+extern "C" __global__ __launch_bounds__(256) void perturbDouble(
+    DeepArgs a, const OrbitEntry<double> *__restrict__ orbit) {
+    perturb_frame<double, 8>(a, orbit);
 }
 extern "C" __global__ __launch_bounds__(256) void deepDirectDoubleExact(DeepArgs a) {
     deep_direct<16, false>(a);

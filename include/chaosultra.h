@@ -363,6 +363,10 @@ int cu_frame_replay(cu_frame f, cu_stream stream);
 #define CU_DEEP_PERTURB_F32 32
 #define CU_DEEP_PERTURB_F64 64
 #define CU_DEEP_REBASE 1u
+/* flags CU_DEEP_REUSE_ORBIT: keep the primary orbit between renders while the
+ * centre and bailout stay the same (it is then computed to the frame's creation
+ * max_iter); results are those of a fresh orbit. */
+#define CU_DEEP_REUSE_ORBIT 2u
 typedef struct cu_deep_desc {
     double centre_x, centre_y;
     double centre_x_lo, centre_y_lo;
@@ -377,7 +381,7 @@ typedef struct cu_deep_desc {
 typedef struct cu_deep_report {
     unsigned long long iterations; /* executed by the last render, all passes */
     uint32_t reference_length;     /* M: last index of the primary reference orbit */
-    uint32_t references;           /* reference orbits computed by the last render */
+    uint32_t references;           /* reference orbits computed by the last render (0: kept) */
     uint32_t glitched;             /* pixels still glitched after the last render */
     uint32_t reserved;
     float orbit_ms;                /* primary reference orbit */

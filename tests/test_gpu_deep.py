@@ -151,3 +151,36 @@ def test_c4_frames_against_double_double(gpu, engine, width):
         print(f"width {width:g} method {method}: mismatch vs double-double "
               f"{100 * rates[method]:.3f} %, {rep.iterations} iterations, {rep.total_ms:.2f} ms")
     assert rates[64] <= 0.01
+
+
+@pytest.mark.parametrize("centre,width", [(CENTRE, 1e-10), ((-0.5, 0.5), 0.3)])
+@pytest.mark.parametrize("method", [32, 64])
+def test_kept_orbit_gives_fresh_results(gpu, engine, method, centre, width):
+    """With CU_DEEP_REUSE_ORBIT a zoom step toward the same centre keeps the orbit
+    (computed to the creation max_iter) and renders what a fresh orbit renders.
+    The second centre is interior: its orbit never escapes, so the kept orbit is
+    longer than a fresh one for the frame's max_iter."""
+    cu = gpu
+
+    def desc(w, mi, flags):
+        d = cu.make_deep(centre[0], centre[1], w, 192, 108, mi, 256.0)
+        d.flags = flags
+        return d
+
+    keep = cu.DEEP_REBASE | cu.DEEP_REUSE_ORBIT
+    f = cu.DeepFrame(engine, desc(width, 4000, keep), method)
+    try:
+        f.render()
+        assert f.report().references == 1
+        f.retarget(desc(width * 0.97, 3200, keep))
+        f.render()
+        rep = f.report()
+        assert rep.references == 0
+        kept = f.counts(), f.smooth()
+    finally:
+        f.close()
+    counts, smooth, _, rep2, _ = render(cu, engine, desc(width * 0.97, 3200, cu.DEEP_REBASE), method)
+    assert rep2.references == 1
+    assert np.array_equal(kept[0], counts)
+    assert np.array_equal(kept[1], smooth)
+    assert counts.any()
