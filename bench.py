@@ -20,6 +20,7 @@ sample grid kernel and the FP32 pipeline on the same frame, and the oracle's CPU
 rate on a bounded row band of the same frame on the host's cores.
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-variants]
+                [--leg uneven|deep|serving]
 """
 from __future__ import annotations
 
@@ -299,6 +300,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the grid / FP32 variants and the uneven-tile and deep-zoom legs")
+    ap.add_argument("--leg", choices=["all", "uneven", "deep", "serving"], default="all",
+                    help="run one leg only and print its JSON (used for profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (a plain copy "
                          "of known size that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -311,6 +314,12 @@ def main():
     cu = importlib.import_module("chaos-ultra_amd")
     cu.init(0)
     engine = cu.Engine()
+    if args.leg != "all":
+        leg = {"uneven": lambda: uneven_leg(cu, engine), "deep": lambda: deep_leg(cu, engine),
+               "serving": lambda: serving_leg(cu)}[args.leg]()
+        engine.close()
+        print(json.dumps({"leg": args.leg, "result": leg}), flush=True)
+        return
     tile = frame_tile(cu)
 
     log(f"[bench] FP64 frame {W}x{H} spp={SPP} maxIter={MAX_ITER} pipeline tune={TUNE}")

@@ -11,8 +11,8 @@
 // double-double (FP64 error-free transformations, ~106-bit significand): an
 // orbit computed in plain FP64 drifts from the true orbit long before the
 // pixel scale of a 1e-20 frame.  It is stored rounded to the delta type,
-// each entry {Zx, Zy, |Z|^2/2, 0}.  Deltas are FP32 or FP64 (same VALU rate on
-// gfx950 without packed FP32; FP64 deltas are far more precise).
+// each entry {Zx, Zy, |Z|^2/2, 0}.  Deltas are FP32 (twice the FP64 issue rate
+// on gfx950) or FP64 (far more precise).
 //
 // Rebasing (Zhuoran): when |z| < |d| — tested as fma(Zx, dx, fma(Zy, dy,
 // |Z|^2/2)) < 0 — or when the pixel outlives the reference (m == M), the pixel
