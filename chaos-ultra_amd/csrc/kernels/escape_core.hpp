@@ -20,6 +20,10 @@
 
 #pragma clang fp contract(off)
 
+// Wave-wide sum over the active lanes (ROCm device library, DPP based).
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_add_u64(
+    unsigned long long);
+
 namespace cu {
 
 // Fused multiply-add with one rounding in the operands' own precision
@@ -122,12 +126,49 @@ __device__ __forceinline__ bool blocks_valid(double bailout_sq, bool fixed_c, do
     return cx * cx + cy * cy <= bailout_sq * 0.999;
 }
 
+// log2 of a positive normal double in about 35 VALU operations.  v = m * 2^e
+// with m in [sqrt(1/2), sqrt(2)); with s = (m - 1) / (m + 1),
+//     ln m = 2 atanh(s) = 2s + sum_{k>=1} 2 s^(2k+1) / (2k + 1),
+// and |s| <= 0.1716 (s^2 <= 0.0295), so the ten terms kept leave a truncation
+// error below 1e-17 relative.  m - 1 is exact; m + 1 and the Newton-refined
+// quotient round once each, so the result is within a few ulp of log2(v).
+// Zero, subnormal, infinite, NaN and negative inputs take the library log2.
+__device__ __forceinline__ double log2_positive(double v) {
+    int e = __builtin_amdgcn_frexp_exp(v);  // v = mant * 2^e, mant in [0.5, 1)
+    double m = __builtin_amdgcn_frexp_mant(v);
+    const bool low = m < 0.70710678118654752440;
+    m = low ? m + m : m;
+    e = low ? e - 1 : e;
+    const double f = m - 1.0;
+    const double d = m + 1.0;
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma_r(r, fma_r(-d, r, 1.0), r);
+    r = fma_r(r, fma_r(-d, r, 1.0), r);
+    double s = f * r;
+    s = fma_r(fma_r(-s, d, f), r, s);
+    const double z = s * s;
+    double p = 2.0 / 21.0;
+    p = fma_r(p, z, 2.0 / 19.0);
+    p = fma_r(p, z, 2.0 / 17.0);
+    p = fma_r(p, z, 2.0 / 15.0);
+    p = fma_r(p, z, 2.0 / 13.0);
+    p = fma_r(p, z, 2.0 / 11.0);
+    p = fma_r(p, z, 2.0 / 9.0);
+    p = fma_r(p, z, 2.0 / 7.0);
+    p = fma_r(p, z, 2.0 / 5.0);
+    p = fma_r(p, z, 2.0 / 3.0);
+    const double ln_m = fma_r(s * z, p, s + s);
+    double out = fma_r(ln_m, 1.44269504088896340736, (double)e);  // log2(e)
+    if (!(v >= 0x1p-1022 && v <= 0x1.fffffffffffffp+1023)) out = log2(v);
+    return out;
+}
+
 // n + 1 - log2(ln(mag2) / ln(R^2)) — the smooth (continuous) escape value —
 // evaluated as n + 1 - (log2(log2(mag2)) - C) with the per-tile constant
 // C = log2(log2(R^2)) (smooth_constant): two log2 and no division per sample.
 // The two forms differ only in rounding, by about 1e-15 before n + 1 is added.
 __device__ __forceinline__ double smooth_value(uint n, double mag2, double smooth_c) {
-    return (double)n + 1.0 - (log2(log2(mag2)) - smooth_c);
+    return (double)n + 1.0 - (log2_positive(log2_positive(mag2)) - smooth_c);
 }
 
 __device__ __forceinline__ double smooth_constant(double bailout_sq) {
@@ -151,8 +192,9 @@ __device__ __forceinline__ void jitter(uint64_t seed, uint64_t sample_id, double
     dy = (double)(uint32_t)h * 0x1.0p-32;
 }
 
-__device__ __forceinline__ void sample_offset(const cu_tile_desc &t, uint px, uint py, uint s,
-                                              double &dx, double &dy) {
+// Offset of the sample numbered `sample_id` = (py * width + px) * spp + s in its tile.
+__device__ __forceinline__ void sample_offset_of(const cu_tile_desc &t, uint64_t sample_id,
+                                                 double &dx, double &dy) {
     if (t.pattern == CU_SAMPLE_CENTRE) {
         dx = 0.5;
         dy = 0.5;
@@ -160,8 +202,13 @@ __device__ __forceinline__ void sample_offset(const cu_tile_desc &t, uint px, ui
         dx = 0.0;
         dy = 0.0;
     } else {
-        jitter(t.seed, ((uint64_t)py * t.width + px) * t.spp + s, dx, dy);
+        jitter(t.seed, sample_id, dx, dy);
     }
+}
+
+__device__ __forceinline__ void sample_offset(const cu_tile_desc &t, uint px, uint py, uint s,
+                                              double &dx, double &dy) {
+    sample_offset_of(t, ((uint64_t)py * t.width + px) * t.spp + s, dx, dy);
 }
 
 // X = left + (px + dx) * pixel_w,  Y = top - (py + dy) * pixel_h

@@ -22,17 +22,12 @@ namespace {
 constexpr uint kMaxLdsTiles = 1024;
 constexpr uint kPaletteLds = 4096;
 
-__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 // The iteration counter is CU_ITER_SLOTS u64 slots; each wave adds its total
 // to one slot chosen by its global wave number, so a launch of a million waves
-// does not serialise on one address.  The caller sums the slots.
+// does not serialise on one address.  The caller sums the slots.  All 64 lanes
+// must be active: the wave sum is the device library's DPP reduction.
 __device__ __forceinline__ void add_iterations(unsigned long long *iters, uint64_t v) {
-    v = wave_sum(v);
+    v = __ockl_wfred_add_u64(v);
     const uint block = blockIdx.x + blockIdx.y * gridDim.x;
     const uint wave = block * ((blockDim.x * blockDim.y + 63u) >> 6) +
                       ((threadIdx.x + threadIdx.y * blockDim.x) >> 6);
@@ -40,12 +35,14 @@ __device__ __forceinline__ void add_iterations(unsigned long long *iters, uint64
         atomicAdd(iters + (wave & (CU_ITER_SLOTS - 1u)), (unsigned long long)v);
 }
 
-// Start sample s of pixel (px, py) of tile t: z0 and c for the tile's kind.
+// Start sample `sample_id` = (py * width + px) * spp + s of tile t (pixel (px, py)):
+// z0 and c for the tile's kind.
 template <class Real>
-__device__ __forceinline__ void begin_sample(const cu_tile_desc &t, uint px, uint py, uint s,
-                                             Real &x, Real &y, Real &cx, Real &cy) {
+__device__ __forceinline__ void begin_sample_of(const cu_tile_desc &t, uint px, uint py,
+                                                uint64_t sample_id, Real &x, Real &y, Real &cx,
+                                                Real &cy) {
     double dx, dy;
-    cu::sample_offset(t, px, py, s, dx, dy);
+    cu::sample_offset_of(t, sample_id, dx, dy);
     Real X, Y;
     cu::tile_point<Real>(t, px, py, dx, dy, X, Y);
     if (t.kind == CU_FRACTAL_JULIA) {
@@ -59,6 +56,13 @@ __device__ __forceinline__ void begin_sample(const cu_tile_desc &t, uint px, uin
         cx = X;
         cy = Y;
     }
+}
+
+// Start sample s of pixel (px, py) of tile t: z0 and c for the tile's kind.
+template <class Real>
+__device__ __forceinline__ void begin_sample(const cu_tile_desc &t, uint px, uint py, uint s,
+                                             Real &x, Real &y, Real &cx, Real &cy) {
+    begin_sample_of<Real>(t, px, py, ((uint64_t)py * t.width + px) * t.spp + s, x, y, cx, cy);
 }
 
 __device__ __forceinline__ bool tile_blocks_valid(const cu_tile_desc &t) {
@@ -379,7 +383,7 @@ extern "C" __global__ __launch_bounds__(256) void checksumU32(const uint *p, uin
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (uint64_t)gridDim.x * blockDim.x)
         acc += (uint64_t)p[k] * (2ull * k + 1ull);
-    acc = wave_sum(acc);
+    acc = __ockl_wfred_add_u64(acc);
     if (cu::lane_id() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
 }
 

@@ -62,14 +62,11 @@ static_assert(sizeof(PipeArgs) == 120, "PipeArgs layout");
 // of serialising on one counter.
 constexpr uint kSubQueues = 1024;
 
-// Finish one sample: checked steps from (x, y, i) until escape or max_iter; its
-// count follows the fractal's computeFractal convention.
+// Checked steps from (x, y, i) until the bailout test fails or i reaches
+// max_iter: the reference loop, for lanes that cannot use an unchecked block.
 template <class Real, bool kFused>
-__device__ __forceinline__ void finish_sample(Real x, Real y, Real cx, Real cy, uint i,
-                                              uint max_iter, Real bsq, double smooth_c, bool julia,
-                                              uint &count, double &smooth, uint64_t &executed) {
-    bool escaped = false;
-    Real mag2 = 0;
+__device__ __forceinline__ void checked_steps(Real x, Real y, Real cx, Real cy, uint &i,
+                                              uint max_iter, Real bsq, bool &escaped, Real &mag2) {
     while (i < max_iter) {
         const Real x2 = x * x;
         const Real y2 = y * y;
@@ -77,7 +74,7 @@ __device__ __forceinline__ void finish_sample(Real x, Real y, Real cx, Real cy, 
         if (!(m < bsq)) {
             escaped = true;
             mag2 = m;
-            break;
+            return;
         }
         const Real xn = (x2 - y2) + cx;
         if constexpr (kFused) {
@@ -88,9 +85,48 @@ __device__ __forceinline__ void finish_sample(Real x, Real y, Real cx, Real cy, 
         x = xn;
         ++i;
     }
-    executed += i;
-    count = escaped ? i : (julia ? max_iter : 0u);
-    smooth = escaped ? cu::smooth_value(i, (double)mag2, smooth_c) : 0.0;
+}
+
+// Exact escape of a lane whose unchecked block from (sx, sy) at iteration i
+// failed its end test, (x, y) being the block's end state and (mx, my) its
+// midpoint.  By escape permanence the first failing test lies in the half whose
+// start is inside: at iteration i0 + k for some k in [0, K/2), or at i0 + K/2
+// itself, where the failing |z|^2 is the one already known (end or midpoint).
+// The K/2 tests run unrolled without branches; every lane steps the same
+// number of times, so the wave spends one straight pass on its pending lanes.
+// Returns n, the iteration of the failing test, and its |z|^2.
+template <class Real, int K, bool kFused>
+__device__ __forceinline__ uint locate_escape(Real x, Real y, Real sx, Real sy, Real mx, Real my,
+                                              Real cx, Real cy, uint i, Real bsq, Real &mag2) {
+    const Real m_end = x * x + y * y;
+    const Real m_mid = mx * mx + my * my;
+    const bool mid_in = m_mid < bsq;
+    Real zx = mid_in ? mx : sx;
+    Real zy = mid_in ? my : sy;
+    const uint i0 = mid_in ? i + (uint)K / 2 : i;
+    uint n = i0 + (uint)K / 2;
+    mag2 = mid_in ? m_end : m_mid;
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < K / 2; ++k) {
+        const Real x2 = zx * zx;
+        const Real y2 = zy * zy;
+        const Real m = x2 + y2;
+        const bool first = !found && !(m < bsq);
+        n = first ? i0 + (uint)k : n;
+        mag2 = first ? m : mag2;
+        found = found || first;
+        if (k + 1 < K / 2) {
+            const Real xn = (x2 - y2) + cx;
+            if constexpr (kFused) {
+                zy = cu::fma_r(zx * zy, (Real)2, cy);
+            } else {
+                zy = ((zx + zx) * zy) + cy;
+            }
+            zx = xn;
+        }
+    }
+    return n;
 }
 
 // Orbit of one lane with the hand-off rule; shared by all passes.  Returns true
@@ -131,10 +167,19 @@ __device__ __forceinline__ bool run_lanes(const PipeArgs &a, bool active, Real x
     uint64_t executed = 0;
     count = 0;
     smooth = 0;
+    bool escaped = false;
+    Real mag2 = 0;
+    if (mode == 2 && replay) {
+        // a failing test at max_iter itself is never reached by the reference loop
+        i = locate_escape<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, bsq, mag2);
+        escaped = i < max_iter;
+    } else if (mode == 2) {
+        checked_steps<Real, kFused>(x, y, cx, cy, i, max_iter, bsq, escaped, mag2);
+    }
     if (mode == 2) {
-        if (replay) cu::replay_start(x, y, i, sx, sy, mx, my, bsq, (uint)K / 2);
-        finish_sample<Real, kFused>(x, y, cx, cy, i, max_iter, bsq, smooth_c, julia, count, smooth,
-                                    executed);
+        executed = i;
+        count = escaped ? i : (julia ? max_iter : 0u);
+        smooth = escaped ? cu::smooth_value(i, (double)mag2, smooth_c) : 0.0;
     }
     // lanes still running: append to this wave's sub-queue (one atomic per wave)
     const bool push = mode == 1;
@@ -160,12 +205,13 @@ __device__ __forceinline__ bool run_lanes(const PipeArgs &a, bool active, Real x
     return mode == 2;
 }
 
-// Pixel outputs from the in-order sums of its samples: integer mean count,
-// mean smooth value and the palette colour L - (round(smooth) mod L) - 1.
-__device__ __forceinline__ void write_pixel(const PipeArgs &a, uint spp, uint64_t o, uint csum,
+// Pixel outputs from the in-order sums of its 2^lspp samples: integer mean
+// count, mean smooth value and the palette colour L - (round(smooth) mod L) - 1.
+// Scaling by 2^-lspp is exact, so it equals the division by spp.
+__device__ __forceinline__ void write_pixel(const PipeArgs &a, uint lspp, uint64_t o, uint csum,
                                             double ssum) {
-    const double sm = ssum / (double)spp;
-    a.counts[o] = csum / spp;
+    const double sm = __builtin_amdgcn_ldexp(ssum, -(int)lspp);
+    a.counts[o] = csum >> lspp;
     if (a.smooth) a.smooth[o] = sm;
     if (a.rgba) {
         const uint vi = (uint)round(sm);
@@ -180,11 +226,14 @@ __device__ __forceinline__ void store_sample(const PipeArgs &a, uint tile, uint 
     a.sample_smooth[so] = smooth;
 }
 
-// Pass 1: wave w of the launch takes wave-block w of the frame.
+// Pass 1: wave w of the launch takes wave-block w of the frame.  The wave
+// number, tile and block geometry are wave-uniform (scalar registers); spp and
+// the block width are powers of two, so lane -> (pixel, sample) is shifts.
 template <class Real, int K, bool kFused>
 __device__ void pass1(const PipeArgs a, uint total_blocks, Straggler<Real> *out_queue,
                       uint *out_count) {
-    const uint wave_global = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint wave_global = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                                            (threadIdx.x >> 6));
     if (wave_global >= total_blocks) return;  // whole waves only: no barrier below
     uint lo = 0, hi = a.n_tiles - 1;          // wave-uniform search of the tile
     while (lo < hi) {
@@ -193,18 +242,20 @@ __device__ void pass1(const PipeArgs a, uint total_blocks, Straggler<Real> *out_
     }
     const uint tile = lo;
     const cu_tile_desc &t = a.tiles[tile];
-    const uint spp = t.spp, lane = cu::lane_id();
+    const uint spp = t.spp, lspp = (uint)__builtin_ctz(spp), lane = cu::lane_id();
     uint bw, bh;
     wave_pixel_block(spp, bw, bh, a.wave_shape);
+    const uint lbw = (uint)__builtin_ctz(bw);
     const uint b = wave_global - a.block_offset[tile];
-    const uint nbx = (t.width + bw - 1) / bw;
-    const uint pix = lane / spp, s = lane % spp;
-    const uint px = (b % nbx) * bw + pix % bw;
-    const uint py = (b / nbx) * bh + pix / bw;
+    const uint nbx = (t.width + bw - 1) >> lbw;
+    const uint bx = b % nbx, by = b / nbx;
+    const uint pix = lane >> lspp, s = lane & (spp - 1u);
+    const uint px = bx * bw + (pix & (bw - 1u));
+    const uint py = by * bh + (pix >> lbw);
     const bool active = px < t.width && py < t.height;
+    const uint sample = ((py * t.width + px) << lspp) | s;
     Real x = 0, y = 0, cx = 0, cy = 0;
-    if (active) begin_sample<Real>(t, px, py, s, x, y, cx, cy);
-    const uint sample = (py * t.width + px) * spp + s;
+    if (active) begin_sample_of<Real>(t, px, py, sample, x, y, cx, cy);
     const uint q = wave_global % kSubQueues;
     uint count;
     double sm;
@@ -229,7 +280,7 @@ __device__ void pass1(const PipeArgs a, uint total_blocks, Straggler<Real> *out_
         if (pending) {
             if (done) store_sample(a, tile, sample, count, sm);
         } else if (s == 0) {
-            write_pixel(a, spp, o, csum, ssum);
+            write_pixel(a, lspp, o, csum, ssum);
         }
         if (s == 0) a.pending[o] = pending ? 1 : 0;
     }
@@ -255,9 +306,9 @@ __device__ void passq_chunk(const PipeArgs &a, const Straggler<Real> *in_queue, 
         sample = e.sample;
         i = e.i;
         const cu_tile_desc &t = a.tiles[tile];
-        const uint pixel = sample / t.spp, s = sample % t.spp;
+        const uint pixel = sample >> __builtin_ctz(t.spp);
         Real zx, zy;
-        begin_sample<Real>(t, pixel % t.width, pixel / t.width, s, zx, zy, cx, cy);
+        begin_sample_of<Real>(t, pixel % t.width, pixel / t.width, sample, zx, zy, cx, cy);
         x = e.x;
         y = e.y;
         max_iter = t.max_iter;
@@ -337,6 +388,7 @@ extern "C" __global__ __launch_bounds__(256) void combinePending(PipeArgs a,
             csum += a.sample_count[so + s];
             ssum += a.sample_smooth[so + s];
         }
-        write_pixel(a, t.spp, t.out_offset + (uint64_t)py * t.out_pitch + px, csum, ssum);
+        write_pixel(a, (uint)__builtin_ctz(t.spp), t.out_offset + (uint64_t)py * t.out_pitch + px,
+                    csum, ssum);
     }
 }
