@@ -157,10 +157,7 @@ def uneven_leg(cu, engine):
     out = {"tiles": len(tiles), "pixels": int(sum(i["side"] ** 2 for i in info)),
            "generate_s": round(gen_s, 1)}
     for name, tn in (("default", cu.FrameTune(16, 16, 5, 0)),
-                     ("tuned_pipeline", cu.FrameTune(res["pipeline"]["best"]["block_steps"],
-                                                     res["pipeline"]["best"]["handoff_lanes"],
-                                                     res["pipeline"]["best"]["tail_passes"],
-                                                     res["pipeline"]["best"]["wave_shape"]))):
+                     ("tuned_pipeline", at.frame_tune(cu, res["pipeline"]["best"]))):
         ms = []
         for _ in range(4):
             frame.render(tn)

@@ -127,7 +127,8 @@ class Tune(C.Structure):
 class FrameTune(C.Structure):
     """cu_frame_tune"""
     _fields_ = [("block_steps", C.c_uint32), ("handoff_lanes", C.c_uint32),
-                ("tail_passes", C.c_uint32), ("wave_shape", C.c_uint32)]
+                ("tail_passes", C.c_uint32), ("wave_shape", C.c_uint32),
+                ("drain_refill", C.c_uint32)]
 
 
 class DeepDesc(C.Structure):

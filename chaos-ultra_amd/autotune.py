@@ -5,7 +5,8 @@ Candidates:
   * the straggler-compaction frame pipeline (cu_frame): pass-1 wave shape (the
     pixel block a wave covers: 8x8 .. 64x1 pixels at 1 spp), unchecked block
     length, hand-off threshold (lanes still running when a wave hands its
-    stragglers to the queue) and number of tail passes;
+    stragglers to the queue), number of tail passes, and whether one persistent
+    drain launch replaces the tail passes (its idle-lane refill threshold);
   * the persistent lane-refill kernel (cu_escape_tiles_persistent): block
     length, resident waves per CU, pixels taken per queue grab and the number
     of idle lanes that trigger a refill.
@@ -26,8 +27,10 @@ PIPELINE_SPACE = {
     "block_steps": (16, 8),
     "handoff_lanes": (8, 16, 24, 32),
     "tail_passes": (3, 5, 7),
+    "drain_refill": (0, 16, 32),
 }
-PIPELINE_START = {"block_steps": 16, "handoff_lanes": 16, "tail_passes": 5, "wave_shape": 0}
+PIPELINE_START = {"block_steps": 16, "handoff_lanes": 16, "tail_passes": 5, "wave_shape": 0,
+                  "drain_refill": 0}
 
 PERSISTENT_SPACE = {
     "block_steps": (16, 8, 32),
@@ -36,6 +39,12 @@ PERSISTENT_SPACE = {
     "refill_lanes": (16, 8, 32),
 }
 PERSISTENT_START = {"block_steps": 16, "waves_per_cu": 32, "chunk": 256, "refill_lanes": 16}
+
+
+def frame_tune(cu, p):
+    """cu.FrameTune from a dict of the pipeline knobs."""
+    return cu.FrameTune(p["block_steps"], p["handoff_lanes"], p["tail_passes"], p["wave_shape"],
+                        p.get("drain_refill", 0))
 
 
 def _median_ms(fn, repeats, warmup):
@@ -98,7 +107,7 @@ def tune_batch(cu, engine, tiles, precision=64, repeats=3, warmup=1, persistent=
     frame = cu.Frame(engine, tiles, precision, -1)
 
     def measure_pipeline(p):
-        tn = cu.FrameTune(p["block_steps"], p["handoff_lanes"], p["tail_passes"], p["wave_shape"])
+        tn = frame_tune(cu, p)
 
         def one():
             frame.render(tn)

@@ -88,6 +88,16 @@ def build_library() -> str:
     return lib
 
 
+def build_replay() -> str:
+    """chaosultra_replay: the native host that replays the reference's call stacks
+    through the C ABI (csrc/harness/replay_calls.cpp)."""
+    exe = os.path.join(OUT_LIB, "chaosultra_replay")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra",
+          os.path.join(ROOT, "chaos-ultra_amd", "csrc", "harness", "replay_calls.cpp"),
+          "-o", exe, f"-L{OUT_LIB}", "-lchaosultra_hip", "-Wl,-rpath,$ORIGIN"])
+    return exe
+
+
 def build_oracle() -> None:
     """The CPU oracle (test infrastructure) and, where the reference tree is present,
     the reference's own iteration functions compiled for the host."""
@@ -99,6 +109,7 @@ def build_all() -> None:
         futs = [ex.submit(build_modules), ex.submit(build_library), ex.submit(build_oracle)]
         for f in futs:
             f.result()
+    build_replay()
 
 
 if __name__ == "__main__":

@@ -39,6 +39,10 @@ static_assert(sizeof(PipeArgs) == 120, "PipeArgs layout");
 constexpr uint32_t kMaxHandoff = 32;   // records a wave may push
 constexpr uint32_t kWaveShapes = 4;    // pixel-block shapes of a pass-1 wave
 constexpr uint32_t kSubQueues = 1024;  // sub-queues per straggler queue (kernel constant)
+constexpr uint32_t kMaxTailPasses = 8;
+// iteration slots (2 x CU_ITER_SLOTS u64), a counter set for pass 1 and each tail
+// pass, and the drain's chunk counter
+constexpr uint64_t kControlWords = 4ull * CU_ITER_SLOTS + (kMaxTailPasses + 1ull) * kSubQueues + 64;
 
 // The bw x bh pixel block of a pass-1 wave (bw * bh = 64 / spp): shape 0 is the
 // near-square block, each further shape twice as wide (capped at one row).
@@ -64,9 +68,13 @@ struct cu_frame_s {
     void *d_sample_count = nullptr, *d_sample_smooth = nullptr;
     void *d_pending = nullptr;   // u8 per pixel
     void *d_queue[2] = {nullptr, nullptr};
-    void *d_counters = nullptr;  // u32[2][kSubQueues] sub-queue fill counts
     void *d_counts = nullptr, *d_smooth = nullptr;
-    void *d_iters = nullptr;     // 2 x CU_ITER_SLOTS u64: pass 1, later passes
+    // Control block, cleared by one fill per render: 2 x CU_ITER_SLOTS u64
+    // iteration slots (pass 1, later passes), then one u32[kSubQueues] set of
+    // sub-queue fill counts per pass (pass 1 writes set 0, tail pass p set p).
+    void *d_control = nullptr;
+    void *d_iters = nullptr;     // = d_control
+    void *d_counters = nullptr;  // = d_control + the iteration slots
     hipEvent_t ev_start = nullptr, ev_pass1 = nullptr, ev_end = nullptr;
     hipStream_t last_stream = nullptr;
     bool rendered = false;
@@ -83,7 +91,7 @@ void *dalloc(size_t bytes) { return device_alloc(bytes); }
 void free_frame(cu_frame f) {
     for (void *p : {f->d_tiles, f->d_block_offset, f->d_sample_offset, f->d_smooth_c,
                     f->d_sample_count, f->d_sample_smooth, f->d_pending, f->d_queue[0],
-                    f->d_queue[1], f->d_counters, f->d_counts, f->d_smooth, f->d_iters})
+                    f->d_queue[1], f->d_counts, f->d_smooth, f->d_control})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : {f->ev_start, f->ev_pass1, f->ev_end})
         if (e) (void)hipEventDestroy(e);
@@ -104,7 +112,8 @@ cu_frame_tune checked_tune(const cu_frame_tune *tune, const uint32_t *d_palette,
         fail(CU_ERR_ILLEGAL_ARGUMENT, "block_steps must be 8 or 16");
     if (tn.handoff_lanes == 0 || tn.handoff_lanes > kMaxHandoff)
         fail(CU_ERR_ILLEGAL_ARGUMENT, "handoff_lanes must be in [1, 32]");
-    if (tn.tail_passes == 0 || tn.tail_passes > 8)
+    if (tn.drain_refill > 64) fail(CU_ERR_ILLEGAL_ARGUMENT, "drain_refill must be in [0, 64]");
+    if (tn.drain_refill == 0 && (tn.tail_passes == 0 || tn.tail_passes > kMaxTailPasses))
         fail(CU_ERR_ILLEGAL_ARGUMENT, "tail_passes must be in [1, 8]");
     if (tn.wave_shape >= kWaveShapes) fail(CU_ERR_ILLEGAL_ARGUMENT, "wave_shape must be in [0, 3]");
     if ((d_rgba == nullptr) != (d_palette == nullptr))
@@ -124,12 +133,12 @@ void enqueue_render(cu_frame f, const cu_frame_tune &tn, const uint32_t *d_palet
                                std::to_string(tn.block_steps);
     cu_function pass1 = function(f->engine, "escapePass1" + suffix);
     cu_function passq = function(f->engine, "escapePassQ" + suffix);
+    cu_function drain = function(f->engine, "escapeDrain" + suffix);
     cu_function combine = function(f->engine, "combinePending");
     uint32_t *counters = (uint32_t *)f->d_counters;
     const uint32_t n_tiles = (uint32_t)f->tiles.size();
 
-    fill(f, stream, f->d_iters, 0, 2 * CU_ITER_SLOTS * 2);
-    fill(f, stream, counters, 0, kSubQueues);
+    fill(f, stream, f->d_control, 0, kControlWords);
     if (events) check(hipEventRecord(f->ev_start, st), "hipEventRecord");
 
     PipeArgs a{(const cu_tile_desc *)f->d_tiles,
@@ -159,23 +168,33 @@ void enqueue_render(cu_frame f, const cu_frame_tune &tn, const uint32_t *d_palet
     }
     if (events) check(hipEventRecord(f->ev_pass1, st), "hipEventRecord");
     const uint32_t cus = (uint32_t)std::max(1, compute_units());
-    for (uint32_t p = 1; p <= tn.tail_passes; ++p) {
+    if (tn.drain_refill) {
+        PipeArgs ap = a;
+        ap.iters = (unsigned long long *)f->d_iters + CU_ITER_SLOTS;
+        const void *q = f->d_queue[0];
+        const uint32_t *fills = counters;
+        uint32_t *grab = counters + (kMaxTailPasses + 1) * kSubQueues;
+        uint32_t refill = tn.drain_refill;
+        void *params[] = {&ap, &q, &fills, &grab, &refill};
+        // 256 threads per block: the kernel's queue prefix assumes it
+        launch(f->engine, drain, dim3(cus * 5), dim3(256), params, stream, false);
+    }
+    for (uint32_t p = 1; !tn.drain_refill && p <= tn.tail_passes; ++p) {
         const uint32_t in = (p - 1) & 1u, outq = p & 1u;
-        fill(f, stream, counters + outq * kSubQueues, 0, kSubQueues);
         PipeArgs ap = a;
         ap.handoff = p == tn.tail_passes ? 0u : tn.handoff_lanes;
         ap.iters = (unsigned long long *)f->d_iters + CU_ITER_SLOTS;
         const void *qin = f->d_queue[in];
-        const uint32_t *cin = counters + in * kSubQueues;
+        const uint32_t *cin = counters + (p - 1) * kSubQueues;
         void *qout = f->d_queue[outq];
-        uint32_t *cout = counters + outq * kSubQueues;
+        uint32_t *cout = counters + p * kSubQueues;
         void *params[] = {&ap, &qin, &cin, &qout, &cout};
         launch(f->engine, passq, dim3(cus * 8), dim3(256), params, stream, false);
     }
     {
         uint64_t total = f->total_pixels;
         void *params[] = {&a, &total};
-        unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, (uint64_t)cus * 32);
+        unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, (uint64_t)cus * 128);
         launch(f->engine, combine, dim3(blocks), dim3(256), params, stream, false);
     }
     if (events) check(hipEventRecord(f->ev_end, st), "hipEventRecord");
@@ -253,10 +272,11 @@ int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, in
         f->d_pending = dalloc(pixels);
         f->d_queue[0] = dalloc(queue_records * rec);
         f->d_queue[1] = dalloc(queue_records * rec);
-        f->d_counters = dalloc(2 * kSubQueues * sizeof(uint32_t));
         f->d_counts = dalloc(pixels * 4);
         f->d_smooth = dalloc(pixels * 8);
-        f->d_iters = dalloc(2 * CU_ITER_SLOTS * 8);
+        f->d_control = dalloc(kControlWords * sizeof(uint32_t));
+        f->d_iters = f->d_control;
+        f->d_counters = (char *)f->d_control + 2 * CU_ITER_SLOTS * 8;
         check(hipMemcpy(f->d_tiles, f->tiles.data(), n_tiles * sizeof(cu_tile_desc),
                         hipMemcpyHostToDevice), "hipMemcpy H2D");
         check(hipMemcpy(f->d_block_offset, block_offset.data(), block_offset.size() * 4,

@@ -129,6 +129,28 @@ __device__ __forceinline__ uint locate_escape(Real x, Real y, Real sx, Real sy, 
     return n;
 }
 
+// Result of a pending lane (its block failed, or it is within K steps of
+// max_iter, or it cannot use blocks): the exact stop, the count in the
+// fractal's computeFractal convention and the smooth value.  i becomes the
+// iterations executed by the sample.
+template <class Real, int K, bool kFused>
+__device__ __forceinline__ void finish_pending(Real x, Real y, Real sx, Real sy, Real mx, Real my,
+                                               Real cx, Real cy, uint &i, uint max_iter, Real bsq,
+                                               double smooth_c, bool julia, bool replay,
+                                               uint &count, double &smooth) {
+    bool escaped = false;
+    Real mag2 = 0;
+    if (replay) {
+        // a failing test at max_iter itself is never reached by the reference loop
+        i = locate_escape<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, bsq, mag2);
+        escaped = i < max_iter;
+    } else {
+        checked_steps<Real, kFused>(x, y, cx, cy, i, max_iter, bsq, escaped, mag2);
+    }
+    count = escaped ? i : (julia ? max_iter : 0u);
+    smooth = escaped ? cu::smooth_value(i, (double)mag2, smooth_c) : 0.0;
+}
+
 // Orbit of one lane with the hand-off rule; shared by all passes.  Returns true
 // when the lane finished its sample (count and smooth set); a lane still
 // running at hand-off is appended to out_queue instead.
@@ -167,19 +189,10 @@ __device__ __forceinline__ bool run_lanes(const PipeArgs &a, bool active, Real x
     uint64_t executed = 0;
     count = 0;
     smooth = 0;
-    bool escaped = false;
-    Real mag2 = 0;
-    if (mode == 2 && replay) {
-        // a failing test at max_iter itself is never reached by the reference loop
-        i = locate_escape<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, bsq, mag2);
-        escaped = i < max_iter;
-    } else if (mode == 2) {
-        checked_steps<Real, kFused>(x, y, cx, cy, i, max_iter, bsq, escaped, mag2);
-    }
     if (mode == 2) {
+        finish_pending<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, max_iter, bsq, smooth_c,
+                                        julia, replay, count, smooth);
         executed = i;
-        count = escaped ? i : (julia ? max_iter : 0u);
-        smooth = escaped ? cu::smooth_value(i, (double)mag2, smooth_c) : 0.0;
     }
     // lanes still running: append to this wave's sub-queue (one atomic per wave)
     const bool push = mode == 1;
@@ -286,11 +299,85 @@ __device__ void pass1(const PipeArgs a, uint total_blocks, Straggler<Real> *out_
     }
 }
 
+// Queue chunks of 64 records numbered densely over the sub-queues: every block
+// copies the fill counts to LDS and scans ceil(fill / 64), so chunk c lies in the
+// last sub-queue q with prefix[q] <= c, as its chunk c - prefix[q].  Needs
+// blockDim.x == 256 (thread t owns sub-queues 4t .. 4t+3); returns the total.
+struct QueueIndex {
+    uint fill[kSubQueues];
+    uint prefix[kSubQueues + 1];
+    uint wave_total[4];
+};
+
+__device__ __forceinline__ uint build_queue_index(QueueIndex &ix, const uint *fill) {
+    const uint t = threadIdx.x, q0 = 4u * t;
+    uint c[4], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint f = fill[q0 + j];
+        ix.fill[q0 + j] = f;
+        c[j] = (f + 63u) >> 6;
+        sum += c[j];
+    }
+    uint incl = sum;  // inclusive scan within the wave
+    const uint ln = cu::lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint v = __shfl_up(incl, off, 64);
+        if (ln >= (uint)off) incl += v;
+    }
+    if (ln == 63) ix.wave_total[t >> 6] = incl;
+    __syncthreads();
+    uint run = incl - sum;
+    for (uint w = 0; w < (t >> 6); ++w) run += ix.wave_total[w];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ix.prefix[q0 + j] = run;
+        run += c[j];
+    }
+    if (t == blockDim.x - 1) ix.prefix[kSubQueues] = run;
+    __syncthreads();
+    return ix.prefix[kSubQueues];
+}
+
+// Sub-queue of dense chunk c (wave-uniform search in LDS).
+__device__ __forceinline__ uint chunk_queue(const QueueIndex &ix, uint c) {
+    uint lo = 0, hi = kSubQueues - 1;
+    while (lo < hi) {
+        const uint mid = (lo + hi + 1) >> 1;
+        if (ix.prefix[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Later passes: a fixed grid strides over the chunks of 64 queued orbits.  Chunk
 // c is chunk c / kSubQueues of sub-queue c % kSubQueues, so the low chunks of
 // every sub-queue come first; a chunk past its sub-queue's fill is skipped.
 // Stragglers of sub-queue q go back to sub-queue q of the output queue, which
 // therefore never holds more than the input sub-queue did.
+// Lane state of a queued orbit: its z, i, tile and sample from the record, c
+// recomputed from the tile and the sample number.
+template <class Real>
+__device__ __forceinline__ void load_straggler(const PipeArgs &a, const Straggler<Real> &e, Real &x,
+                                               Real &y, Real &cx, Real &cy, uint &i, uint &tile,
+                                               uint &sample, uint &max_iter, Real &bsq,
+                                               double &smooth_c, bool &julia, bool &blocks) {
+    tile = e.tile;
+    sample = e.sample;
+    i = e.i;
+    const cu_tile_desc &t = a.tiles[tile];
+    const uint pixel = sample >> __builtin_ctz(t.spp);
+    Real zx, zy;
+    begin_sample_of<Real>(t, pixel % t.width, pixel / t.width, sample, zx, zy, cx, cy);
+    x = e.x;
+    y = e.y;
+    max_iter = t.max_iter;
+    bsq = (Real)t.bailout_sq;
+    smooth_c = a.smooth_c[tile];
+    julia = t.kind == CU_FRACTAL_JULIA;
+    blocks = tile_blocks_valid(t);
+}
+
 template <class Real, int K, bool kFused>
 __device__ void passq_chunk(const PipeArgs &a, const Straggler<Real> *in_queue, uint n, uint chunk,
                             Straggler<Real> *out_queue, uint *out_count) {
@@ -300,23 +387,9 @@ __device__ void passq_chunk(const PipeArgs &a, const Straggler<Real> *in_queue, 
     uint i = 0, tile = 0, sample = 0, max_iter = 0;
     double smooth_c = 1;
     bool julia = false, blocks = false;
-    if (active) {
-        const Straggler<Real> e = in_queue[idx];
-        tile = e.tile;
-        sample = e.sample;
-        i = e.i;
-        const cu_tile_desc &t = a.tiles[tile];
-        const uint pixel = sample >> __builtin_ctz(t.spp);
-        Real zx, zy;
-        begin_sample_of<Real>(t, pixel % t.width, pixel / t.width, sample, zx, zy, cx, cy);
-        x = e.x;
-        y = e.y;
-        max_iter = t.max_iter;
-        bsq = (Real)t.bailout_sq;
-        smooth_c = a.smooth_c[tile];
-        julia = t.kind == CU_FRACTAL_JULIA;
-        blocks = tile_blocks_valid(t);
-    }
+    if (active)
+        load_straggler<Real>(a, in_queue[idx], x, y, cx, cy, i, tile, sample, max_iter, bsq,
+                             smooth_c, julia, blocks);
     uint count;
     double sm;
     if (run_lanes<Real, K, kFused>(a, active, x, y, cx, cy, i, max_iter, bsq, smooth_c, julia,
@@ -327,16 +400,115 @@ __device__ void passq_chunk(const PipeArgs &a, const Straggler<Real> *in_queue, 
 template <class Real, int K, bool kFused>
 __device__ void passq(const PipeArgs a, const Straggler<Real> *in_queue, const uint *in_count,
                       Straggler<Real> *out_queue, uint *out_count) {
-    const uint wave_global = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    __shared__ QueueIndex ix;
+    const uint chunks = build_queue_index(ix, in_count);
+    const uint wave_global = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                                            (threadIdx.x >> 6));
     const uint total_waves = gridDim.x * (blockDim.x >> 6);
-    const uint chunks = kSubQueues * ((a.subq_capacity + 63u) / 64u);
     for (uint c = wave_global; c < chunks; c += total_waves) {
-        const uint q = c % kSubQueues, k = c / kSubQueues;
-        const uint n = in_count[q];
-        if (k * 64u >= n) continue;  // wave-uniform
+        const uint q = chunk_queue(ix, c), k = c - ix.prefix[q];
         const uint64_t region = (uint64_t)q * a.subq_capacity;
-        passq_chunk<Real, K, kFused>(a, in_queue + region, n, k, out_queue + region, out_count + q);
+        passq_chunk<Real, K, kFused>(a, in_queue + region, ix.fill[q], k, out_queue + region,
+                                     out_count + q);
     }
+}
+
+// Drain: one persistent launch runs every orbit left in the pass-1 queue to its
+// end, refilling idle lanes from the queue (the lane-refill scheme of
+// escape_tiles_persistent applied to queued orbits), so the longest orbits
+// start at once instead of after several hand-offs.  A wave takes one dense
+// queue chunk (build_queue_index) per atomic on `grab` and hands its records to
+// idle lanes.  Once `refill` lanes wait, the
+// pending lanes finish (exact stop, smooth value, per-sample store) and idle
+// lanes take new records.  Every wave leaves when the chunks are exhausted and
+// its lanes are idle; there is no barrier after the prefix.
+template <class Real, int K, bool kFused>
+__device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint *fill,
+                      uint *grab, uint refill) {
+    __shared__ QueueIndex ix;
+    const uint total_chunks = build_queue_index(ix, fill);
+    const uint lane = cu::lane_id();
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+
+    int mode = 0;  // 0 idle, 1 running unchecked blocks, 2 pending
+    bool replay = false;
+    Real x = 0, y = 0, cx = 0, cy = 0, sx = 0, sy = 0, mx = 0, my = 0, bsq = 4;
+    uint i = 0, tile = 0, sample = 0, max_iter = 0;
+    double smooth_c = 1;
+    bool julia = false, blocks = false;
+    uint64_t executed = 0;
+
+    uint cq = 0, cur = 0, end = 0;  // wave-uniform record cursor in sub-queue cq
+    bool exhausted = false;
+
+    for (;;) {
+        // ---- refill idle lanes ----
+        uint64_t idle = cu::ballot(mode == 0);
+        while (idle != 0 && !exhausted) {
+            if (cur >= end) {
+                uint c = 0;
+                if (lane == 0) c = atomicAdd(grab, 1u);
+                c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+                if (c >= total_chunks) {
+                    exhausted = true;
+                    break;
+                }
+                cq = chunk_queue(ix, c);
+                cur = (c - ix.prefix[cq]) * 64u;
+                end = min(cur + 64u, ix.fill[cq]);
+            }
+            const uint n = (uint)__popcll(idle);
+            const uint take = min(n, end - cur);
+            const uint rank = (uint)__popcll(idle & lanes_below);
+            if (mode == 0 && rank < take) {
+                const Straggler<Real> e = queue[(uint64_t)cq * a.subq_capacity + cur + rank];
+                load_straggler<Real>(a, e, x, y, cx, cy, i, tile, sample, max_iter, bsq, smooth_c,
+                                     julia, blocks);
+                replay = false;
+                mode = (blocks && max_iter - i >= (uint)K) ? 1 : 2;
+            }
+            cur += take;
+            idle = cu::ballot(mode == 0);
+        }
+        if (cu::ballot(mode != 0) == 0) break;
+
+        // ---- unchecked blocks until enough lanes wait ----
+        for (;;) {
+            if (mode == 1) {
+                sx = x;
+                sy = y;
+#pragma unroll
+                for (int k = 0; k < K / 2; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+                mx = x;
+                my = y;
+#pragma unroll
+                for (int k = K / 2; k < K; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+                if (x * x + y * y < bsq) {
+                    i += K;
+                    if (max_iter - i < (uint)K) mode = 2;
+                } else {
+                    mode = 2;
+                    replay = true;
+                }
+            }
+            const uint64_t run = cu::ballot(mode == 1);
+            const uint waiting = (uint)__popcll(cu::ballot(mode == 2)) +
+                                 (exhausted ? 0u : (uint)__popcll(cu::ballot(mode == 0)));
+            if (run == 0 || waiting >= refill) break;
+        }
+
+        // ---- pending lanes finish their sample ----
+        if (mode == 2) {
+            uint count;
+            double sm;
+            finish_pending<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, max_iter, bsq,
+                                            smooth_c, julia, replay, count, sm);
+            executed += i;
+            store_sample(a, tile, sample, count, sm);
+            mode = 0;
+        }
+    }
+    add_iterations(a.iters, executed);
 }
 
 }  // namespace
@@ -350,6 +522,10 @@ __device__ void passq(const PipeArgs a, const Straggler<Real> *in_queue, const u
         PipeArgs a, const Straggler<REAL> *in_queue, const uint *in_count,                        \
         Straggler<REAL> *out_queue, uint *out_count) {                                            \
         passq<REAL, STEPS, FUSED>(a, in_queue, in_count, out_queue, out_count);                   \
+    }                                                                                             \
+    extern "C" __global__ __launch_bounds__(256) void escapeDrain##NAME##K##STEPS(                \
+        PipeArgs a, const Straggler<REAL> *queue, const uint *fill, uint *grab, uint refill) {    \
+        drain<REAL, STEPS, FUSED>(a, queue, fill, grab, refill);                                  \
     }
 
 CU_PIPE_KERNELS(DoubleExact, double, false, 8)
@@ -380,7 +556,6 @@ extern "C" __global__ __launch_bounds__(256) void combinePending(PipeArgs a,
         }
         const cu_tile_desc &t = a.tiles[lo];
         const uint local = (uint)(p - lds_offset[lo]);
-        const uint px = local % t.width, py = local / t.width;
         const uint64_t so = a.sample_offset[lo] + (uint64_t)local * t.spp;
         uint csum = 0;
         double ssum = 0;
@@ -388,7 +563,9 @@ extern "C" __global__ __launch_bounds__(256) void combinePending(PipeArgs a,
             csum += a.sample_count[so + s];
             ssum += a.sample_smooth[so + s];
         }
-        write_pixel(a, (uint)__builtin_ctz(t.spp), t.out_offset + (uint64_t)py * t.out_pitch + px,
-                    csum, ssum);
+        uint64_t o = t.out_offset + local;  // rows packed: pitch == width
+        if (t.out_pitch != t.width)
+            o = t.out_offset + (uint64_t)(local / t.width) * t.out_pitch + local % t.width;
+        write_pixel(a, (uint)__builtin_ctz(t.spp), o, csum, ssum);
     }
 }

@@ -319,6 +319,9 @@ typedef struct cu_frame_tune {
     uint32_t tail_passes;   /* 1..8 */
     uint32_t wave_shape;    /* 0..3: pass-1 wave covers a bw x bh pixel block, bw * bh = 64/spp;
                                0 near-square (spp 1: 8x8), each step twice as wide (16x4, 32x2, 64x1) */
+    uint32_t drain_refill;  /* 0: the queue runs through tail_passes launches; 1..64: one persistent
+                               drain launch runs it to the end, refilling a wave's idle lanes once
+                               this many wait (tail_passes is then ignored) */
 } cu_frame_tune;
 typedef struct cu_frame_s *cu_frame;
 int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, int precision,

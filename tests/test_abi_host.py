@@ -152,3 +152,13 @@ def test_init_without_device_is_initialization_error(cu):
         pytest.skip("a device is visible")
     with pytest.raises(cu.CudaInitializationError):
         cu.init(0)
+
+
+def test_replay_host_built(cu):
+    """The native call-stack replay host is built beside the library and checks
+    its arguments before touching a device."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(cu.abi.LIB_PATH), "chaosultra_replay")
+    assert os.access(exe, os.X_OK)
+    p = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=30)
+    assert p.returncode == 2 and "usage" in p.stdout

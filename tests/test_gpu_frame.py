@@ -1,8 +1,9 @@
 """The straggler-compaction frame pipeline on the GPU against the oracle.
 
-Whatever the hand-off threshold, block length and number of tail passes, every
-sample must come out exactly as the oracle's sequential loop computes it: escape
-counts and iteration totals bit-exact, smooth values within 1e-12 relative.
+Whatever the hand-off threshold, block length, number of tail passes or the
+persistent drain of the queue, every sample must come out exactly as the
+oracle's sequential loop computes it: escape counts and iteration totals
+bit-exact, smooth values within 1e-12 relative.
 """
 import hashlib
 import json
@@ -36,13 +37,17 @@ def render(cu, engine, tiles, precision, variant, tune, palette=None):
         f.close()
 
 
-@pytest.mark.parametrize("tune", [(16, 16, 3), (8, 16, 3), (16, 1, 1), (16, 32, 8), (8, 4, 2)])
+# (block_steps, handoff_lanes, tail_passes, drain_refill): drain_refill > 0 runs
+# the queue in one persistent launch instead of tail passes
+@pytest.mark.parametrize("tune", [(16, 16, 3, 0), (8, 16, 3, 0), (16, 1, 1, 0), (16, 32, 8, 0),
+                                  (8, 4, 2, 0), (16, 24, 0, 16), (8, 32, 0, 64), (16, 8, 0, 1)])
 @pytest.mark.parametrize("precision", [64, 32])
 def test_c2_frame_pipeline_vs_oracle(gpu, engine, oracle, tune, precision):
     cu = gpu
     seg = cu.plane_segment_from_center(*C2_CENTER, 1e-4 / 8, 512, 384)
     tile = cu.make_tile(*seg, 512, 384, 5000, 256.0, spp=4, pattern=cu.SAMPLE_JITTER, seed=3)
-    counts, smooth, it = render(cu, engine, [tile], precision, -1, cu.FrameTune(*tune, 0))
+    b, h, p, d = tune
+    counts, smooth, it = render(cu, engine, [tile], precision, -1, cu.FrameTune(b, h, p, 0, d))
     want_c, want_s, want_it = oracle.escape_tile(tile, precision)
     assert np.array_equal(counts.reshape(384, 512), want_c)
     assert it == want_it
@@ -65,11 +70,12 @@ def test_mixed_tiles_pipeline_vs_oracle(gpu, engine, oracle, precision):
     tiles, total = mixed_tiles(cu, rng, 40, max_side=120)
     for t in tiles:  # the pipeline needs spp a power of two
         t.spp = [1, 2, 4][t.spp % 3]
-    counts, smooth, it = render(cu, engine, tiles, precision, 0, cu.FrameTune(16, 8, 4, 0))
     want_c, want_s, want_it = oracle_tiles(oracle, tiles, total, precision)
-    assert np.array_equal(counts, want_c)
-    assert it == want_it
-    assert_smooth_close(smooth, want_s)
+    for tune in (cu.FrameTune(16, 8, 4, 0), cu.FrameTune(8, 16, 0, 2, 24)):
+        counts, smooth, it = render(cu, engine, tiles, precision, 0, tune)
+        assert np.array_equal(counts, want_c)
+        assert it == want_it
+        assert_smooth_close(smooth, want_s)
 
 
 def test_fused_compose(gpu, engine, oracle):
@@ -93,6 +99,10 @@ def test_frame_rejects(gpu, engine):
     try:
         with pytest.raises(cu.IllegalArgumentError, match="handoff_lanes"):
             f.render(cu.FrameTune(16, 40, 2, 0))
+        with pytest.raises(cu.IllegalArgumentError, match="tail_passes"):
+            f.render(cu.FrameTune(16, 16, 0, 0, 0))
+        with pytest.raises(cu.IllegalArgumentError, match="drain_refill"):
+            f.render(cu.FrameTune(16, 16, 0, 0, 65))
         with pytest.raises(cu.IllegalStateError):
             f.iterations()
     finally:

@@ -134,8 +134,6 @@ def test_autotuner_pick_renders_the_same_pixels(gpu, engine):
     assert res["engine"] in ("pipeline", "persistent") and res["ms"] > 0
     assert len(res["pipeline"]["table"]) >= 10
     p = res["pipeline"]["best"]
-    got, _, _ = render_frame(cu, engine, tiles,
-                             cu.FrameTune(p["block_steps"], p["handoff_lanes"], p["tail_passes"],
-                                          p["wave_shape"]))
+    got, _, _ = render_frame(cu, engine, tiles, at.frame_tune(cu, p))
     want, _, _ = render_frame(cu, engine, tiles, cu.FrameTune(16, 16, 5, 0))
     assert np.array_equal(got, want)
