@@ -175,6 +175,23 @@ def palette_from_argb(argb) -> np.ndarray:
     return out
 
 
+def load_palette(path: str | None, max_entries: int = 1 << 16):
+    """ImageHelpers.loadColorPaletteOrDefault: (RGBA palette, loaded-from-file flag)."""
+    out = np.zeros(max(max_entries, 1536), dtype=np.uint32)
+    n, loaded = C.c_size_t(), C.c_int()
+    abi.call("cu_palette_load_or_default", (path or "").encode(),
+             out.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_size_t(out.size), C.byref(n),
+             C.byref(loaded))
+    return out[:n.value].copy(), bool(loaded.value)
+
+
+def save_png(path: str, rgba: np.ndarray) -> None:
+    """ImageHelpers.saveImageToFile(..., "png") of an (h, w) array of RGBA words."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint32)
+    h, w = rgba.shape
+    abi.call("cu_image_save_png", path.encode(), rgba.ctypes.data_as(C.POINTER(C.c_uint32)), w, h)
+
+
 class FractalRenderer:
     """FractalRenderer (rendering/FractalRenderer.java) backed by a native renderer handle."""
 

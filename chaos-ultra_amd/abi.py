@@ -245,6 +245,9 @@ SIGNATURES = {
     "cu_launch_geometry": (_I, [_I, _I, C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
     "cu_default_palette": (_I, [C.POINTER(_U32), _SZ]),
     "cu_palette_from_argb": (_I, [C.POINTER(_U32), C.POINTER(_U32), _SZ]),
+    "cu_palette_load_or_default": (_I, [C.c_char_p, C.POINTER(_U32), _SZ, C.POINTER(_SZ),
+                                        C.POINTER(C.c_int)]),
+    "cu_image_save_png": (_I, [C.c_char_p, C.POINTER(_U32), C.c_int, C.c_int]),
     "cu_tile_init": (_I, [C.POINTER(TileDesc), _D, _D, _D, _D, _U32, _U32, _U32, _D, _U32, _D, _D,
                           _U32, _U32, _U64]),
     "cu_tile_fused_safe": (_I, [C.POINTER(TileDesc), _I, C.POINTER(_I)]),

@@ -39,7 +39,8 @@ HOST_FLAGS = [
     "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off",
     "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
 ]
-HOST_SOURCES = ("driver.cpp", "renderer.cpp", "engine.cpp", "frame.cpp", "deep.cpp", "serve.cpp")
+HOST_SOURCES = ("driver.cpp", "renderer.cpp", "engine.cpp", "frame.cpp", "deep.cpp", "serve.cpp",
+                "png_io.cpp")
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:
@@ -80,7 +81,7 @@ def build_library() -> str:
         objs = list(ex.map(compile_one, HOST_SOURCES))
     lib = os.path.join(OUT_LIB, "libchaosultra_hip.so")
     tmp = lib + ".tmp"
-    _run(["g++", "-shared", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64",
+    _run(["g++", "-shared", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", "-lz",
           f"-Wl,-rpath,{ROCM}/lib", "-Wl,--no-undefined"])
     os.replace(tmp, lib)
     for o in objs:

@@ -200,6 +200,20 @@ int cu_launch_geometry(int width, int height, unsigned *grid_x, unsigned *grid_y
 int cu_default_palette(uint32_t *out, size_t count);
 /* ImageHelpers.fromBGRAtoRGBA (util/ImageHelpers.java:118-130) */
 int cu_palette_from_argb(const uint32_t *argb, uint32_t *rgba, size_t count);
+/* ImageHelpers.loadColorPaletteOrDefault / loadColorPaletteFromFile
+ * (util/ImageHelpers.java:43-73): the first row of the PNG at `path` as RGBA words
+ * (getRGB's ARGB through fromBGRAtoRGBA), or the 1536-entry default palette when
+ * `path` is NULL or empty, does not exist, or is not a readable PNG (the reference
+ * falls back on an IOException; a file ImageIO cannot identify, on which the
+ * reference would fail with a NullPointerException, also falls back here).
+ * *count = entries written; *loaded (may be NULL) = 1 if the file was used.
+ * CU_ERR_ILLEGAL_ARGUMENT if `cap` is too small. */
+int cu_palette_load_or_default(const char *path, uint32_t *out, size_t cap, size_t *count,
+                               int *loaded);
+/* ImageHelpers.saveImageToFile(rgbs, w, h, name, "png") (util/ImageHelpers.java:192-214):
+ * an 8-bit RGB PNG of width*height RGBA words, R from the least significant byte,
+ * alpha dropped; missing parent directories are created. */
+int cu_image_save_png(const char *path, const uint32_t *rgba, int width, int height);
 
 /* ------------------------------------------------------------------------- */
 /* 3. escape-family layer                                                    */

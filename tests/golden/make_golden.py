@@ -8,8 +8,9 @@ Sources of truth:
     oracle/_ref/libref.so).  Only input/output vectors are written: full-frame
     SHA-256 digests of the uint32 escape counts plus sampled (px, py, count) rows,
     and per-point outputs.
-  * palettes/*.png of the reference (image data), as ARGB words the way
-    java.awt.image.BufferedImage.getRGB returns them.
+  * palettes/*.png of the reference (image data): copied to tests/golden/palettes/
+    as inputs of the palette loader, and their first rows as ARGB words the way
+    java.awt.image.BufferedImage.getRGB returns them (decoded by PIL).
 """
 from __future__ import annotations
 
@@ -113,10 +114,14 @@ def main():
     with open(os.path.join(HERE, "reference_iteration.json"), "w") as f:
         json.dump(out, f, separators=(",", ":"))
 
-    # palettes (image data of the reference)
+    # palettes (image data of the reference): the PNG files as loader inputs, and
+    # their first rows decoded by PIL as the expected ARGB words
     from PIL import Image
+    import shutil
     pal = {}
+    os.makedirs(os.path.join(HERE, "palettes"), exist_ok=True)
     for fn in sorted(os.listdir(os.path.join(REFERENCE, "palettes"))):
+        shutil.copyfile(os.path.join(REFERENCE, "palettes", fn), os.path.join(HERE, "palettes", fn))
         im = Image.open(os.path.join(REFERENCE, "palettes", fn)).convert("RGB")
         w, _ = im.size
         argb = [(0xFF << 24) | (r << 16) | (g << 8) | b for r, g, b in (im.getpixel((x, 0)) for x in range(w))]
