@@ -47,7 +47,7 @@ SEED = 20261202
 FLOPS_PER_ITER = 8
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-TUNE = (16, 16, 5)  # block_steps, handoff_lanes, tail_passes
+TUNE = (16, 24, 3)  # block_steps, handoff_lanes, tail_passes
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_fp64_pass1_traffic.md")
 
 

@@ -106,7 +106,7 @@ void fill(cu_frame f, cu_stream s, void *p, uint32_t value, uint64_t words) {
 
 cu_frame_tune checked_tune(const cu_frame_tune *tune, const uint32_t *d_palette,
                            uint32_t palette_length, const uint32_t *d_rgba) {
-    cu_frame_tune tn{16, 16, 5, 0};
+    cu_frame_tune tn{16, 24, 3, 0, 0};
     if (tune) tn = *tune;
     if (tn.block_steps != 8 && tn.block_steps != 16)
         fail(CU_ERR_ILLEGAL_ARGUMENT, "block_steps must be 8 or 16");
