@@ -169,8 +169,11 @@ void launch_pixels(cu_deep d, cu_stream s, DeepArgs &a) {
     cu_function f = function(d->engine, name);
     const void *orbit = a.orbit;
     void *params[] = {&a, &orbit};  // direct kernels take only the first
-    dim3 grid((d->desc.width + 15) / 16, (d->desc.height + 15) / 16, 1);
-    launch(d->engine, f, grid, dim3(256), params, s, false);
+    // direct: 256 threads = 16 x 16 pixels; perturbation: 1024 threads = 32 x 32
+    // pixels sharing one LDS copy of the reference orbit
+    const uint32_t side = d->method == CU_DEEP_DIRECT_F64 ? 16 : 32;
+    dim3 grid((d->desc.width + side - 1) / side, (d->desc.height + side - 1) / side, 1);
+    launch(d->engine, f, grid, dim3(side * side), params, s, false);
 }
 
 // The glitched pixel nearest the centroid of all glitched pixels.

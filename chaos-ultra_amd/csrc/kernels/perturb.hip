@@ -27,7 +27,8 @@
 // checked steps — the same operation sequence as the oracle's checked loop.
 //
 //   referenceOrbit{Float,Double}   one lane: double-double orbit of C -> entries, M
-//   perturb{Float,Double}          lane per pixel (8x8 pixels per wave)
+//   perturb{Float,Double}          lane per pixel (8x8 pixels per wave, 32x32 per block,
+//                                  orbit staged in LDS)
 //   deepDirectDouble               direct FP64 at the same pixel -> c mapping
 
 namespace {
@@ -64,6 +65,19 @@ __device__ __forceinline__ DD dd_mul(DD a, DD b) {
 template <class Real>
 struct OrbitEntry {
     Real x, y, h, pad;  // Z_m and |Z_m|^2 / 2
+};
+
+// The reference orbit as the pixel kernels read it: entries below `staged`
+// from the block's LDS copy, the rest (an orbit longer than the LDS budget)
+// from global memory.
+template <class Real>
+struct OrbitView {
+    const OrbitEntry<Real> *global;
+    const OrbitEntry<Real> *lds;
+    uint staged;
+    __device__ __forceinline__ OrbitEntry<Real> at(uint m) const {
+        return m < staged ? lds[m] : global[m];
+    }
 };
 
 template <class Real>
@@ -129,10 +143,10 @@ __device__ __forceinline__ void deep_offset(const DeepArgs &a, uint px, uint py,
 // One checked step of the perturbation loop (the oracle's or_perturb_pixel
 // body).  Returns 0 to continue, 1 escaped (mag set), 2 max_iter, 3 glitched.
 template <class Real>
-__device__ __forceinline__ int checked_step(const OrbitEntry<Real> *Z, uint M, Real &dx, Real &dy,
+__device__ __forceinline__ int checked_step(const OrbitView<Real> &Z, uint M, Real &dx, Real &dy,
                                             uint &m, uint &n, Real dcx, Real dcy, uint max_iter,
                                             Real bsq, bool rebase, Real geps, Real &mag_out) {
-    const OrbitEntry<Real> e = Z[m];
+    const OrbitEntry<Real> e = Z.at(m);
     Real Zx = e.x, Zy = e.y;
     const Real zx = Zx + dx;
     const Real zy = Zy + dy;
@@ -195,10 +209,11 @@ __device__ __forceinline__ uint64_t unchecked_block(const OrbitEntry<Real> *zp, 
 
 // Whole pixel; lanes whose `active` is false only take part in the ballots.
 // When every lane running a block reads the same orbit entries (same m), the
-// block reads them once per wave through the scalar cache instead of once per
-// lane through the vector memory path.
+// block reads them once per wave through the scalar cache; otherwise (lanes
+// rebased at different steps) each lane reads its own entries from the LDS
+// copy, or from global memory past it.
 template <class Real, int K>
-__device__ int perturb_pixel(const OrbitEntry<Real> *Z, uint M, Real dcx, Real dcy, uint max_iter,
+__device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dcy, uint max_iter,
                              Real bsq, bool rebase, Real geps, bool active, uint &n_out,
                              Real &mag_out) {
     Real dx = 0, dy = 0;
@@ -214,12 +229,16 @@ __device__ int perturb_pixel(const OrbitEntry<Real> *Z, uint M, Real dcx, Real d
             const Real sx = dx, sy = dy;
             const uint sm = m, sn = n;
             const uint mu = __builtin_amdgcn_readfirstlane(m);
-            const uint64_t flagged = cu::ballot(m != mu) == 0
-                                         ? unchecked_block<Real, K>(Z + mu, dx, dy, dcx, dcy)
-                                         : unchecked_block<Real, K>(Z + m, dx, dy, dcx, dcy);
+            uint64_t flagged;
+            if (cu::ballot(m != mu) == 0)
+                flagged = unchecked_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy);
+            else if (m + (uint)K <= Z.staged)
+                flagged = unchecked_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy);
+            else
+                flagged = unchecked_block<Real, K>(Z.global + m, dx, dy, dcx, dcy);
             m += K;
             n += K;
-            const OrbitEntry<Real> e = Z[m];
+            const OrbitEntry<Real> e = Z.at(m);
             const Real zx = e.x + dx;
             const Real zy = e.y + dy;
             const Real mag = cu::fma_r(zx, zx, zy * zy);
@@ -255,17 +274,28 @@ __device__ __forceinline__ void deep_write(const DeepArgs &a, uint64_t o, int st
     }
 }
 
-// 256 threads = 2 x 2 waves of 8 x 8 pixels.
+// A block of W x W waves, each wave 8 x 8 pixels (256 threads: W = 2; 1024: W = 4).
+template <uint W>
 __device__ __forceinline__ void deep_pixel_coords(uint &px, uint &py) {
     const uint lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    px = (blockIdx.x * 2u + (wave & 1u)) * 8u + (lane & 7u);
-    py = (blockIdx.y * 2u + (wave >> 1)) * 8u + (lane >> 3);
+    px = (blockIdx.x * W + (wave % W)) * 8u + (lane & 7u);
+    py = (blockIdx.y * W + (wave / W)) * 8u + (lane >> 3);
 }
+
+// Bytes of reference orbit each perturbation block stages in LDS (4096 FP32 /
+// 2048 FP64 entries); two 1024-thread blocks share a CU's 160 KB.
+constexpr uint kOrbitLdsBytes = 64u * 1024u;
 
 template <class Real, int K>
 __device__ void perturb_frame(const DeepArgs a, const OrbitEntry<Real> *__restrict__ orbit) {
+    __shared__ OrbitEntry<Real> lds[kOrbitLdsBytes / sizeof(OrbitEntry<Real>)];
+    const uint M = *a.orbit_len;
+    const uint staged = min(M + 1u, kOrbitLdsBytes / (uint)sizeof(OrbitEntry<Real>));
+    for (uint k = threadIdx.x; k < staged; k += blockDim.x) lds[k] = orbit[k];
+    __syncthreads();
+    const OrbitView<Real> view{orbit, lds, staged};
     uint px, py;
-    deep_pixel_coords(px, py);
+    deep_pixel_coords<4>(px, py);
     const uint64_t o = (uint64_t)py * a.width + px;
     bool active = px < a.width && py < a.height;
     if (active && a.only_glitched) active = a.glitch[o] != 0;
@@ -276,11 +306,10 @@ __device__ void perturb_frame(const DeepArgs a, const OrbitEntry<Real> *__restri
         dcx = (Real)(ox - a.ref_ox);
         dcy = (Real)(oy - a.ref_oy);
     }
-    const uint M = *a.orbit_len;
     uint n = 0;
     Real mag = 0;
     const int state = perturb_pixel<Real, K>(
-        orbit, M, dcx, dcy, a.max_iter, (Real)a.bailout_sq,
+        view, M, dcx, dcy, a.max_iter, (Real)a.bailout_sq,
         (a.flags & kDeepRebase) != 0, (Real)a.glitch_eps, active, n, mag);
     if (active) deep_write(a, o, state, n, (double)mag);
     add_iterations(a.iters, active ? (uint64_t)n : 0u);
@@ -290,7 +319,7 @@ __device__ void perturb_frame(const DeepArgs a, const OrbitEntry<Real> *__restri
 template <int K, bool kFused>
 __device__ void deep_direct(const DeepArgs a) {
     uint px, py;
-    deep_pixel_coords(px, py);
+    deep_pixel_coords<2>(px, py);
     const bool active = px < a.width && py < a.height;
     uint64_t executed = 0;
     if (active) {
@@ -318,12 +347,13 @@ extern "C" __global__ __launch_bounds__(64) void referenceOrbitDouble(
     reference_orbit<double>(cx_hi, cx_lo, cy_hi, cy_lo, max_iter, bailout_sq, orbit, length);
 }
 // `orbit` repeats a.orbit as its own read-only argument, so uniform loads from
-// it can go through the scalar cache.
-extern "C" __global__ __launch_bounds__(256) void perturbFloat(
+// it can go through the scalar cache.  1024 threads: 32 x 32 pixels per block
+// share one LDS copy of the orbit.
+extern "C" __global__ __launch_bounds__(1024) void perturbFloat(
     DeepArgs a, const OrbitEntry<float> *__restrict__ orbit) {
     perturb_frame<float, 8>(a, orbit);
 }
-extern "C" __global__ __launch_bounds__(256) void perturbDouble(
+extern "C" __global__ __launch_bounds__(1024) void perturbDouble(
     DeepArgs a, const OrbitEntry<double> *__restrict__ orbit) {
     perturb_frame<double, 8>(a, orbit);
 }
