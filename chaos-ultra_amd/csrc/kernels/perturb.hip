@@ -230,15 +230,19 @@ __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dc
             const uint sm = m, sn = n;
             const uint mu = __builtin_amdgcn_readfirstlane(m);
             uint64_t flagged;
-            if (cu::ballot(m != mu) == 0)
+            OrbitEntry<Real> e;
+            if (cu::ballot(m != mu) == 0) {
                 flagged = unchecked_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy);
-            else if (m + (uint)K <= Z.staged)
-                flagged = unchecked_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy);
-            else
-                flagged = unchecked_block<Real, K>(Z.global + m, dx, dy, dcx, dcy);
+                e = Z.global[mu + (uint)K];
+            } else {
+                if (m + (uint)K <= Z.staged)
+                    flagged = unchecked_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy);
+                else
+                    flagged = unchecked_block<Real, K>(Z.global + m, dx, dy, dcx, dcy);
+                e = Z.at(m + (uint)K);
+            }
             m += K;
             n += K;
-            const OrbitEntry<Real> e = Z.at(m);
             const Real zx = e.x + dx;
             const Real zy = e.y + dy;
             const Real mag = cu::fma_r(zx, zx, zy * zy);
