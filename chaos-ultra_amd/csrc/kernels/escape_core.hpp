@@ -47,6 +47,23 @@ __device__ __forceinline__ void quad_step(Real &x, Real &y, Real cx, Real cy) {
     x = xn;
 }
 
+// One step with the squares carried: (x2, y2) hold x*x and y*y of the current
+// state on entry and of the new state on exit, so a block-end test reads them
+// instead of recomputing them.  Same operations and roundings as quad_step.
+template <class Real, bool kFused>
+__device__ __forceinline__ void quad_step_sq(Real &x, Real &y, Real &x2, Real &y2, Real cx,
+                                             Real cy) {
+    const Real xn = (x2 - y2) + cx;
+    if constexpr (kFused) {
+        y = fma_r(x * y, (Real)2, cy);
+    } else {
+        y = ((x + x) * y) + cy;
+    }
+    x = xn;
+    x2 = x * x;
+    y2 = y * y;
+}
+
 // Where the replay of a failed block starts: at its midpoint (iteration
 // i + half) when that state is still inside, else at its start.
 template <class Real>

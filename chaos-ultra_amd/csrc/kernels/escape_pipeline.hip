@@ -164,27 +164,36 @@ __device__ __forceinline__ bool run_lanes(const PipeArgs &a, bool active, Real x
     int mode = !active ? 0 : ((blocks && max_iter - i >= (uint)K) ? 1 : 2);
     bool replay = false;
     Real sx = x, sy = y, mx = x, my = y;
-    for (;;) {
-        const uint64_t run = cu::ballot(mode == 1);
-        if (run == 0) break;
-        if (a.handoff && (uint)__popcll(run) <= a.handoff) break;
+    Real x2 = x * x, y2 = y * y;          // squares of the current state, carried
+    const uint last = max_iter - (uint)K;  // a block may start at i <= last
+    auto block = [&]() {
         if (mode == 1) {
             sx = x;
             sy = y;
 #pragma unroll
-            for (int k = 0; k < K / 2; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+            for (int k = 0; k < K / 2; ++k) cu::quad_step_sq<Real, kFused>(x, y, x2, y2, cx, cy);
             mx = x;
             my = y;
 #pragma unroll
-            for (int k = K / 2; k < K; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
-            if (x * x + y * y < bsq) {
+            for (int k = K / 2; k < K; ++k) cu::quad_step_sq<Real, kFused>(x, y, x2, y2, cx, cy);
+            if (x2 + y2 < bsq) {
                 i += K;
-                if (max_iter - i < (uint)K) mode = 2;
+                if (i > last) mode = 2;
             } else {
                 mode = 2;
                 replay = true;
             }
         }
+    };
+    auto keep_going = [&]() {
+        const uint64_t run = cu::ballot(mode == 1);
+        return run != 0 && !(a.handoff && (uint)__popcll(run) <= a.handoff);
+    };
+    // two blocks per trip: the state registers return to place without copies
+    while (keep_going()) {
+        block();
+        if (!keep_going()) break;
+        block();
     }
     uint64_t executed = 0;
     count = 0;
@@ -537,11 +546,63 @@ CU_PIPE_KERNELS(FloatExact, float, false, 16)
 CU_PIPE_KERNELS(FloatFused, float, true, 8)
 CU_PIPE_KERNELS(FloatFused, float, true, 16)
 
+namespace {
+
+// combinePending for a frame of one tile with 4 samples per pixel (the planar
+// frame): a thread takes 4 consecutive pixels, reads their 4 pending flags as
+// one word and, for the pending ones, issues every per-sample load (one 16-byte
+// load of counts, two of smooth values per pixel) before using any, so the
+// kernel waits for one round of loads per 4 pixels instead of one per pixel.
+// Flag words are 4-byte aligned (the plane starts on an allocation boundary),
+// and so are the 16-byte sample groups (sample_offset[0] = 0, 4 per pixel).
+__device__ void combine_one_tile_spp4(const PipeArgs &a, uint64_t total_pixels) {
+    const cu_tile_desc &t = a.tiles[0];
+    const uint64_t soff = a.sample_offset[0];
+    const uint *flags4 = reinterpret_cast<const uint *>(a.pending);
+    const uint4 *counts4 = reinterpret_cast<const uint4 *>(a.sample_count + soff);
+    const double2 *smooth2 = reinterpret_cast<const double2 *>(a.sample_smooth + soff);
+    const uint64_t groups = total_pixels / 4;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint f = flags4[g];
+        if (f == 0) continue;
+        uint4 c[4];
+        double2 s0[4], s1[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if ((f >> (8 * j)) & 0xffu) {
+                const uint64_t p = g * 4 + j;
+                c[j] = counts4[p];
+                s0[j] = smooth2[2 * p];
+                s1[j] = smooth2[2 * p + 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if ((f >> (8 * j)) & 0xffu) {
+                const uint64_t local = g * 4 + j;
+                const uint csum = c[j].x + c[j].y + c[j].z + c[j].w;
+                const double ssum = ((s0[j].x + s0[j].y) + s1[j].x) + s1[j].y;
+                uint64_t o = t.out_offset + local;
+                if (t.out_pitch != t.width)
+                    o = t.out_offset + (uint64_t)(local / t.width) * t.out_pitch + local % t.width;
+                write_pixel(a, 2u, o, csum, ssum);
+            }
+        }
+    }
+}
+
+}  // namespace
+
 // Pixels whose samples finished in several passes: sum the per-sample planes in
 // sample order and write the pixel outputs.  Thread per frame pixel; pixel p lies
 // in the last tile whose work_offset (pixel prefix) is <= p.
 extern "C" __global__ __launch_bounds__(256) void combinePending(PipeArgs a,
                                                                  uint64_t total_pixels) {
+    if (a.n_tiles == 1 && a.tiles[0].spp == 4 && total_pixels % 4 == 0) {
+        combine_one_tile_spp4(a, total_pixels);  // wave-uniform: every thread of the launch
+        return;
+    }
     __shared__ uint64_t lds_offset[kMaxLdsTiles];
     for (uint k = threadIdx.x; k < a.n_tiles; k += blockDim.x)
         lds_offset[k] = a.tiles[k].work_offset;

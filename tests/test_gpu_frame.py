@@ -107,3 +107,22 @@ def test_frame_rejects(gpu, engine):
             f.iterations()
     finally:
         f.close()
+
+
+@pytest.mark.parametrize("spp,w,h", [(4, 101, 37), (8, 64, 40), (16, 48, 24), (64, 16, 12), (1, 77, 33)])
+def test_pipeline_spp_and_ragged_sizes(gpu, engine, oracle, spp, w, h):
+    """Every power-of-two spp, and frames whose pixel count is not a multiple of
+    4 (the combine's general path) or whose width is not a multiple of the wave
+    block, against the oracle; hand-off after the first lanes finish so that many
+    pixels are completed by the tail passes and the combine."""
+    cu = gpu
+    seg = cu.plane_segment_from_center(*C2_CENTER, 1e-4, w, h)
+    tile = cu.make_tile(*seg, w, h, 3000, 256.0, spp=spp, pattern=cu.SAMPLE_JITTER, seed=spp)
+    want_c, want_s, want_it = oracle.escape_tile(tile, 64)
+    pal = ((np.arange(256, dtype=np.uint32) * 0x00010305) | 0xFF000000).astype(np.uint32)
+    for tune in (cu.FrameTune(16, 32, 2, 0), cu.FrameTune(8, 16, 0, 0, 8)):
+        counts, smooth, it, rgba = render(cu, engine, [tile], 64, -1, tune, palette=pal)
+        assert np.array_equal(counts.reshape(h, w), want_c)
+        assert it == want_it
+        assert_smooth_close(smooth.reshape(h, w), want_s)
+        assert np.array_equal(rgba.reshape(h, w), oracle.compose_smooth(want_s, pal))
