@@ -35,6 +35,12 @@ bool read_file(const std::string &path, std::vector<uint8_t> &out) {
     return true;
 }
 
+// Bounds on what a palette file may ask the reader to allocate: the first row
+// holds at most kMaxWidth entries, and an interlaced file is inflated up to its
+// sixth pass (where row 0 ends) only while that stays below kMaxInflate bytes.
+constexpr uint32_t kMaxWidth = 1u << 24;
+constexpr size_t kMaxInflate = size_t(1) << 30;
+
 struct Png {
     uint32_t width = 0, height = 0;
     int depth = 0, ctype = 0, interlace = 0;
@@ -98,6 +104,7 @@ bool parse(const std::vector<uint8_t> &b, Png &png) {
     }
     // like ImageIO's reader, a stream that ends after its image data needs no IEND
     if (!ihdr || png.idat.empty() || png.width == 0 || png.height == 0) return false;
+    if (png.width > kMaxWidth) return false;  // no palette is this wide; refuse the allocation
     if (!depth_allowed(png.ctype, png.depth)) return false;
     if (png.ctype == 3 && (png.plte.empty() || png.plte.size() % 3)) return false;
     return true;
@@ -231,8 +238,13 @@ bool first_row_argb(const std::string &path, std::vector<uint32_t> &argb) {
         offset[p] = total;
         if (pw[p] && ph) total += ph * (1 + line_bytes(pw[p]));
     }
+    const size_t need = pw[5] ? offset[5] + 1 + line_bytes(pw[5])
+                      : pw[3] ? offset[3] + 1 + line_bytes(pw[3])
+                      : pw[1] ? offset[1] + 1 + line_bytes(pw[1])
+                              : 1 + line_bytes(pw[0]);
+    if (need > kMaxInflate) return false;
     std::vector<uint8_t> raw, row;
-    if (!inflate_all(png.idat, raw, total)) return false;
+    if (!inflate_all(png.idat, raw, need)) return false;
     for (int p : {0, 1, 3, 5}) {
         if (!pw[p]) continue;
         if (raw.size() < offset[p] + 1 + line_bytes(pw[p])) return false;

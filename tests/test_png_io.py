@@ -152,3 +152,16 @@ def test_save_png_round_trip(cu, tmp_path):
     assert loaded and np.array_equal(pal, (rgba[0] & 0xFFFFFF) | 0xFF000000)
     with pytest.raises(cu.IllegalArgumentError):
         cu.save_png(str(tmp_path / "x.png"), np.zeros((0, 4), dtype=np.uint32))
+
+
+@pytest.mark.parametrize("w,h,interlace", [(1 << 25, 1, 0), (64, (1 << 31) - 1, 1)])
+def test_oversized_headers_fall_back(cu, tmp_path, w, h, interlace):
+    """A header asking for a huge first row, or an interlaced image whose first
+    row lies past a huge inflate, is refused (default palette) before anything
+    of that size is allocated."""
+    png = (b"\x89PNG\r\n\x1a\n" + png_chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, interlace))
+           + png_chunk(b"IDAT", zlib.compress(b"\x00" * 64)) + png_chunk(b"IEND", b""))
+    path = tmp_path / "big.png"
+    path.write_bytes(png)
+    pal, loaded = cu.load_palette(str(path))
+    assert not loaded and np.array_equal(pal, cu.default_palette())
