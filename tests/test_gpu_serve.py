@@ -168,3 +168,31 @@ def test_serve_rejects(gpu):
             server.submit(sid, (-0.75, 0.1), 1e-3, 1000, 0)
     finally:
         server.close()
+
+
+def test_deep_frames_with_fp64_deltas(gpu):
+    """deep_method CU_DEEP_PERTURB_F64: a served deep frame equals the oracle's
+    FP64-delta perturbation (bailout 4, rebasing) + smooth compose, and a second
+    frame of the session (narrower, same centre) is still exact."""
+    cu = gpu
+    server = sv.Server(cu.KERNELS_DIR, 1 << 30, slots=2, deep_method=cu.DEEP_PERTURB_F64)
+    pal = sv.palette256()
+    centre = (-0.7436438870, 0.1318259042)
+    try:
+        sid = server.open(80, 48, pal, 2500)
+        for k, w in enumerate((6e-13, 6e-13 * sv.ZOOM)):
+            server.submit(sid, centre, w, 2500, k)
+            done = wait_frame(server)[0]
+            assert done.path == 2 and done.iterations > 0
+            got = server.read_rgba(sid, 80, 48)
+            d = cu.make_deep(centre[0], centre[1], w, 80, 48, 2500, 4.0)
+            orbit, M = O.reference_orbit(centre[0], centre[1], 2500, 4.0, 64)
+            ys, xs = np.mgrid[0:48, 0:80]
+            c, mag, st, _ = O.deep_pixels(O.DEEP_PERTURB_F64, xs.ravel(), ys.ravel(), d, orbit, M)
+            sm = np.where(st == 1, c + 1.0 - np.log2(np.log(np.where(st == 1, mag, 4.0)) / math.log(4.0)),
+                          0.0)
+            assert np.array_equal(got, O.compose_smooth(sm.reshape(48, 80), pal)), f"frame {k}"
+    finally:
+        server.close()
+    with pytest.raises(cu.IllegalArgumentError, match="deep_method"):
+        sv.Server(cu.KERNELS_DIR, 1 << 30, slots=1, deep_method=cu.DEEP_DIRECT_F64)
