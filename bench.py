@@ -304,7 +304,7 @@ def main():
                          "of known size that scales the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; this build has no multi-GPU path (asked for "
+        print(f"bench.py measures one GPU: the render path has no multi-GPU form (asked for "
               f"--gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
