@@ -164,8 +164,8 @@ __device__ void escape_tile_samples(const cu_tile_desc &t, double smooth_c, uint
 // Every wave keeps 64 independent samples in flight.  Lanes run unchecked
 // blocks of K steps in lock step; a lane whose block fails its end test (or
 // whose remaining budget is below K) becomes pending.  Once `refill` lanes are
-// pending or idle, the pending lanes replay their last block step by step (the
-// exact escape step), finish their sample, start the next sample of the same
+// pending or idle, the pending lanes locate the exact escape step in the half of
+// their last block that holds it (cu::locate_escape), finish their sample, start the next sample of the same
 // pixel or write the pixel, and idle lanes take new pixels from a wave-private
 // chunk of the global pixel queue (one atomic per `chunk` pixels).  Pixels of
 // one tile are numbered row-major from the tile's work_offset.  There is no
@@ -188,10 +188,10 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
     const uint64_t lanes_below = (1ull << lane) - 1ull;
 
     int mode = 0;          // 0 idle, 1 running unchecked blocks, 2 pending checked work
-    bool replay = false;   // pending lane restarts its failed block from (sx, sy)
+    bool replay = false;   // pending lane's last block failed: locate the escape in it
     uint tile = 0, px = 0, py = 0, s = 0, i = 0, count_sum = 0;
     double smooth_sum = 0;
-    Real x = 0, y = 0, cx = 0, cy = 0, sx = 0, sy = 0, bsq = 4;
+    Real x = 0, y = 0, cx = 0, cy = 0, sx = 0, sy = 0, mx = 0, my = 0, bsq = 4;
     uint max_iter = 0;
     uint64_t executed = 0;
 
@@ -226,9 +226,9 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
                 }
                 tile = lo;
                 const cu_tile_desc &t = tiles[tile];
-                const uint64_t local = item - lds_offset[tile];
-                px = (uint)(local % t.width);
-                py = (uint)(local / t.width);
+                const uint local = (uint)(item - lds_offset[tile]);  // a tile holds < 2^32 pixels
+                px = local % t.width;
+                py = local / t.width;
                 s = 0;
                 count_sum = 0;
                 smooth_sum = 0;
@@ -250,7 +250,11 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
                 sx = x;
                 sy = y;
 #pragma unroll
-                for (int k = 0; k < K; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+                for (int k = 0; k < K / 2; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+                mx = x;
+                my = y;
+#pragma unroll
+                for (int k = K / 2; k < K; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
                 if (x * x + y * y < bsq) {
                     i += K;
                     if (max_iter - i < (uint)K) mode = 2;
@@ -265,32 +269,17 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
             if (run == 0 || waiting >= refill) break;
         }
 
-        // ---- pending lanes: checked steps, finish the sample ----
+        // ---- pending lanes: locate the escape or run checked steps, finish the sample ----
         if (mode == 2) {
-            if (replay) {
-                x = sx;
-                y = sy;
-                replay = false;
-            }
             bool escaped = false;
             Real mag2 = 0;
-            while (i < max_iter) {
-                const Real x2 = x * x;
-                const Real y2 = y * y;
-                const Real m = x2 + y2;
-                if (!(m < bsq)) {
-                    escaped = true;
-                    mag2 = m;
-                    break;
-                }
-                const Real xn = (x2 - y2) + cx;
-                if constexpr (kFused) {
-                    y = cu::fma_r(x * y, (Real)2, cy);
-                } else {
-                    y = ((x + x) * y) + cy;
-                }
-                x = xn;
-                ++i;
+            if (replay) {
+                // a failing test at max_iter itself is never reached by the reference loop
+                i = cu::locate_escape<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, bsq, mag2);
+                escaped = i < max_iter;
+                replay = false;
+            } else {
+                cu::checked_steps<Real, kFused>(x, y, cx, cy, i, max_iter, bsq, escaped, mag2);
             }
             const cu_tile_desc &t = tiles[tile];
             executed += i;

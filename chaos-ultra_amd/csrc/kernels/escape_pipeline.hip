@@ -62,73 +62,6 @@ static_assert(sizeof(PipeArgs) == 120, "PipeArgs layout");
 // of serialising on one counter.
 constexpr uint kSubQueues = 1024;
 
-// Checked steps from (x, y, i) until the bailout test fails or i reaches
-// max_iter: the reference loop, for lanes that cannot use an unchecked block.
-template <class Real, bool kFused>
-__device__ __forceinline__ void checked_steps(Real x, Real y, Real cx, Real cy, uint &i,
-                                              uint max_iter, Real bsq, bool &escaped, Real &mag2) {
-    while (i < max_iter) {
-        const Real x2 = x * x;
-        const Real y2 = y * y;
-        const Real m = x2 + y2;
-        if (!(m < bsq)) {
-            escaped = true;
-            mag2 = m;
-            return;
-        }
-        const Real xn = (x2 - y2) + cx;
-        if constexpr (kFused) {
-            y = cu::fma_r(x * y, (Real)2, cy);
-        } else {
-            y = ((x + x) * y) + cy;
-        }
-        x = xn;
-        ++i;
-    }
-}
-
-// Exact escape of a lane whose unchecked block from (sx, sy) at iteration i
-// failed its end test, (x, y) being the block's end state and (mx, my) its
-// midpoint.  By escape permanence the first failing test lies in the half whose
-// start is inside: at iteration i0 + k for some k in [0, K/2), or at i0 + K/2
-// itself, where the failing |z|^2 is the one already known (end or midpoint).
-// The K/2 tests run unrolled without branches; every lane steps the same
-// number of times, so the wave spends one straight pass on its pending lanes.
-// Returns n, the iteration of the failing test, and its |z|^2.
-template <class Real, int K, bool kFused>
-__device__ __forceinline__ uint locate_escape(Real x, Real y, Real sx, Real sy, Real mx, Real my,
-                                              Real cx, Real cy, uint i, Real bsq, Real &mag2) {
-    const Real m_end = x * x + y * y;
-    const Real m_mid = mx * mx + my * my;
-    const bool mid_in = m_mid < bsq;
-    Real zx = mid_in ? mx : sx;
-    Real zy = mid_in ? my : sy;
-    const uint i0 = mid_in ? i + (uint)K / 2 : i;
-    uint n = i0 + (uint)K / 2;
-    mag2 = mid_in ? m_end : m_mid;
-    bool found = false;
-#pragma unroll
-    for (int k = 0; k < K / 2; ++k) {
-        const Real x2 = zx * zx;
-        const Real y2 = zy * zy;
-        const Real m = x2 + y2;
-        const bool first = !found && !(m < bsq);
-        n = first ? i0 + (uint)k : n;
-        mag2 = first ? m : mag2;
-        found = found || first;
-        if (k + 1 < K / 2) {
-            const Real xn = (x2 - y2) + cx;
-            if constexpr (kFused) {
-                zy = cu::fma_r(zx * zy, (Real)2, cy);
-            } else {
-                zy = ((zx + zx) * zy) + cy;
-            }
-            zx = xn;
-        }
-    }
-    return n;
-}
-
 // Result of a pending lane (its block failed, or it is within K steps of
 // max_iter, or it cannot use blocks): the exact stop, the count in the
 // fractal's computeFractal convention and the smooth value.  i becomes the
@@ -142,10 +75,10 @@ __device__ __forceinline__ void finish_pending(Real x, Real y, Real sx, Real sy,
     Real mag2 = 0;
     if (replay) {
         // a failing test at max_iter itself is never reached by the reference loop
-        i = locate_escape<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, bsq, mag2);
+        i = cu::locate_escape<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, bsq, mag2);
         escaped = i < max_iter;
     } else {
-        checked_steps<Real, kFused>(x, y, cx, cy, i, max_iter, bsq, escaped, mag2);
+        cu::checked_steps<Real, kFused>(x, y, cx, cy, i, max_iter, bsq, escaped, mag2);
     }
     count = escaped ? i : (julia ? max_iter : 0u);
     smooth = escaped ? cu::smooth_value(i, (double)mag2, smooth_c) : 0.0;
