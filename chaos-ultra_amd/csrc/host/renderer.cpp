@@ -385,7 +385,9 @@ struct RenderKernel {
 // grid checks of launchRenderingKernel / launchDrawingKernel (:242-253, :305-331);
 // returns false for an empty image (nothing is launched).
 bool grid_for(int w, int h, unsigned *gx, unsigned *gy) {
-    int x = (w + kBlockDim - 1) / kBlockDim, y = (h + kBlockDim - 1) / kBlockDim;
+    // in 64 bits: a width near INT_MAX must not overflow on the way to the check
+    const int64_t x = ((int64_t)w + kBlockDim - 1) / kBlockDim;
+    const int64_t y = ((int64_t)h + kBlockDim - 1) / kBlockDim;
     if (x <= 0 || y <= 0) return false;
     if (x > kMaxGridDim)
         fail(CU_ERR_RENDERER, "Unsupported input parameter: width must be smaller than " +

@@ -97,6 +97,8 @@ def test_launch_geometry_limits(cu):
         cu.launch_geometry(65535 * 32 + 1, 1)
     with pytest.raises(cu.FractalRendererError, match="height must be smaller than"):
         cu.launch_geometry(1, 65535 * 32 + 1)
+    with pytest.raises(cu.FractalRendererError, match="width"):  # no int overflow on the way
+        cu.launch_geometry(2 ** 31 - 1, 32)
 
 
 def test_default_palette_matches_png_and_oracle(cu, oracle):
