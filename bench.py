@@ -11,7 +11,8 @@ interior sample counts maxIter).
 
 Also reported (variants): the 256-tile uneven batch (autotuned, captured
 replay), 16 zoom sessions through the continuous-batching loop (p50/p99
-arrival-to-completion latency), the 1920x1080 deep-zoom frames (FP32 and FP64-delta perturbation and
+arrival-to-completion latency), the drop-in FractalRenderer path at 3840x2160
+(renderQuality and renderFast), the 1920x1080 deep-zoom frames (FP32 and FP64-delta perturbation and
 direct FP64 with their mismatch against the double-double oracle subset), and
 the roofline of the dominant kernel (pass 1 of the pipeline:
 8 algorithmic FP64 FLOPs per iteration it executes, against the 78.6 TFLOP/s
@@ -20,7 +21,7 @@ sample grid kernel and the FP32 pipeline on the same frame, and the oracle's CPU
 rate on a bounded row band of the same frame on the host's cores.
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-variants]
-                [--leg uneven|deep|serving]
+                [--leg uneven|deep|serving|renderer]
 """
 from __future__ import annotations
 
@@ -233,6 +234,54 @@ def deep_leg(cu, engine):
     return out
 
 
+def renderer_leg(cu, w=3840, h=2160, frames=5):
+    """The drop-in path (FractalRenderer): renderQuality on the module's default
+    view (supplyDefaultValues: centre (-0.5, 0), zoom 2, maxIterations 1600,
+    maxSuperSampling 5, adaptive) at 3840x2160, then renderFast frames zooming in
+    by 0.977 about the centre with reuse and foveation.  Iterations are the
+    module's executed-iteration count (cu_renderer_iterations)."""
+    prov = cu.FractalRendererProvider()
+    r = prov.get_default_renderer()
+    try:
+        r.initialize_rendering(w, h, cu.default_palette())
+        m = cu.RenderingModel()
+        r.supply_default_values(m, w, h)
+        m.use_adaptive_super_sampling = 1
+        m.use_sample_reuse = 1
+        m.use_foveated_rendering = 1
+        r.render_quality(m)  # warm-up
+        out = {"view": "default (centre -0.5,0; zoom 2)", "size": f"{w}x{h}",
+               "max_iter": int(m.max_iterations), "max_super_sampling": float(m.max_super_sampling)}
+        ms, its = [], []
+        for _ in range(frames):
+            i0, t0 = r.iterations(), time.perf_counter()
+            r.render_quality(m)
+            ms.append((time.perf_counter() - t0) * 1e3)
+            its.append(r.iterations() - i0)
+        k = int(np.argsort(ms)[len(ms) // 2])
+        out["render_quality"] = {"ms_p50": round(ms[k], 3), "iterations": int(its[k]),
+                                 "giga_iter_s": round(its[k] / ms[k] / 1e6, 1)}
+        ms, its = [], []
+        cx = 0.5 * (m.left_bottom_x + m.right_top_x)
+        cy = 0.5 * (m.left_bottom_y + m.right_top_y)
+        for _ in range(frames * 4):
+            hw = 0.5 * (m.right_top_x - m.left_bottom_x) * 0.977
+            hh = 0.5 * (m.right_top_y - m.left_bottom_y) * 0.977
+            m.set_segment(cx - hw, cy - hh, cx + hw, cy + hh)
+            m.zooming, m.zooming_in = 1, 1
+            m.mouse_focus_x, m.mouse_focus_y = w // 2, h // 2
+            i0, t0 = r.iterations(), time.perf_counter()
+            r.render_fast(m)
+            ms.append((time.perf_counter() - t0) * 1e3)
+            its.append(r.iterations() - i0)
+        out["render_fast"] = {"frames": len(ms), "ms_p50": round(float(np.median(ms)), 3),
+                              "giga_iter_s": round(float(np.sum(its)) / float(np.sum(ms)) / 1e6, 1)}
+        r.free_rendering_resources()
+        return out
+    finally:
+        prov.close()
+
+
 def serving_leg(cu, hbm_budget=16 << 30, slots=8):
     """16 zoom sessions, Poisson arrivals at 30 fps each, 120 frames, replayed in
     real time through the continuous-batching loop."""
@@ -297,7 +346,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the grid / FP32 variants and the uneven-tile and deep-zoom legs")
-    ap.add_argument("--leg", choices=["all", "uneven", "deep", "serving"], default="all",
+    ap.add_argument("--leg", choices=["all", "uneven", "deep", "serving", "renderer"], default="all",
                     help="run one leg only and print its JSON (used for profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (a plain copy "
@@ -313,7 +362,7 @@ def main():
     engine = cu.Engine()
     if args.leg != "all":
         leg = {"uneven": lambda: uneven_leg(cu, engine), "deep": lambda: deep_leg(cu, engine),
-               "serving": lambda: serving_leg(cu)}[args.leg]()
+               "serving": lambda: serving_leg(cu), "renderer": lambda: renderer_leg(cu)}[args.leg]()
         engine.close()
         print(json.dumps({"leg": args.leg, "result": leg}), flush=True)
         return
@@ -357,6 +406,8 @@ def main():
         log("[bench] serving: 16 zoom sessions in real time")
         variants["serving_16_sessions"] = serving_leg(cu)
         log(f"[bench] serving: {variants['serving_16_sessions']}")
+        variants["renderer_4k"] = renderer_leg(cu)
+        log(f"[bench] renderer (drop-in path) 3840x2160: {variants['renderer_4k']}")
         log("[bench] deep zoom 1920x1080")
         variants["deep_zoom_1080p"] = deep_leg(cu, engine)
         log(f"[bench] deep zoom: {json.dumps(variants['deep_zoom_1080p'])}")

@@ -245,6 +245,12 @@ class FractalRenderer:
         abi.call("cu_renderer_supply_default_values", self.handle, C.byref(model), canvas_width,
                  canvas_height)
 
+    def iterations(self) -> int:
+        """Iterations executed by this renderer's module since it was loaded."""
+        n = C.c_ulonglong()
+        abi.call("cu_renderer_iterations", self.handle, C.byref(n))
+        return n.value
+
     def read_output(self) -> np.ndarray:
         w, h = self.get_width(), self.get_height()
         out = np.zeros(w * h, dtype=np.uint32)

@@ -237,6 +237,7 @@ SIGNATURES = {
     "cu_renderer_set_fractal_custom_params": (_I, [C.c_int32, C.c_char_p]),
     "cu_renderer_get_fractal_name": (_I, [C.c_int32, C.c_char_p, _SZ]),
     "cu_renderer_supply_default_values": (_I, [C.c_int32, C.POINTER(RenderingModel), _I, _I]),
+    "cu_renderer_iterations": (_I, [_I, C.POINTER(C.c_ulonglong)]),
     "cu_renderer_read_output": (_I, [C.c_int32, C.POINTER(_U32), _SZ]),
     "cu_renderer_read_pixel_info": (_I, [C.c_int32, _I, _P, _SZ]),
     "cu_set_debug_mode": (_I, [_I]),

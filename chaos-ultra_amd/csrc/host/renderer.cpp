@@ -163,6 +163,21 @@ class Module {
         initialized_ = false;
     }
 
+    // Sum of the module's cu_iteration_slots (iterations executed since load).
+    unsigned long long iterations() const {
+        if (!initialized_) fail(CU_ERR_ILLEGAL_STATE, "Module has not been initialized.");
+        void *d = nullptr;
+        size_t bytes = 0;
+        int rc = cu_module_get_global(module_, "cu_iteration_slots", &d, &bytes);
+        if (rc != CU_OK) fail(rc, cu_last_error());
+        std::vector<unsigned long long> slots(bytes / sizeof(unsigned long long));
+        rc = cu_memcpy_dtoh(slots.data(), d, slots.size() * sizeof(unsigned long long));
+        if (rc != CU_OK) fail(rc, cu_last_error());
+        unsigned long long total = 0;
+        for (unsigned long long v : slots) total += v;
+        return total;
+    }
+
     void write_constant(const std::string &sym, const void *v, size_t bytes) {
         if (!initialized_) fail(CU_ERR_ILLEGAL_STATE, "Module has not been initialized.");
         int rc = cu_module_write_global(module_, sym.c_str(), v, bytes);
@@ -803,6 +818,13 @@ int cu_renderer_supply_default_values(cu_renderer r, cu_rendering_model *m, int 
         renderer(r)->module()->supply_default_values(*m, w, h);
     });
 }
+int cu_renderer_iterations(cu_renderer r, unsigned long long *total) {
+    return locked([&] {
+        if (!total) fail(CU_ERR_ILLEGAL_ARGUMENT, "total is NULL");
+        *total = renderer(r)->module()->iterations();
+    });
+}
+
 int cu_renderer_read_output(cu_renderer r, uint32_t *rgba, size_t count) {
     return locked([&] {
         if (!rgba) fail(CU_ERR_ILLEGAL_ARGUMENT, "rgba is NULL");

@@ -178,6 +178,12 @@ int cu_renderer_supply_default_values(cu_renderer r, cu_rendering_model *model, 
                                       int canvas_height);
 /* Read back the composed RGBA8 image (width*height words, row-major). */
 int cu_renderer_read_output(cu_renderer r, uint32_t *rgba, size_t count);
+/* Iterations executed by the renderer's fractal module since it was loaded: the
+ * sum of its cu_iteration_slots (each sample adds the loop bodies computeFractal
+ * ran — the escape step, or maxIterations for an orbit that never escapes).  The
+ * reference has no counterpart; a caller takes differences around a render to
+ * get its Giga-iterations/s. */
+int cu_renderer_iterations(cu_renderer r, unsigned long long *total);
 /* Read back the primary (1) or secondary (0) pixel_info_t buffer, rows packed
  * (width*16 bytes each). */
 int cu_renderer_read_pixel_info(cu_renderer r, int primary, void *dst, size_t bytes);

@@ -30,6 +30,7 @@
 #include "chaos_oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -68,6 +69,11 @@ uint32_t escape_loop(uint32_t max_iter, Real x, Real y, Real cx, Real cy, Real b
     return i;
 }
 
+// Iterations executed by compute_fractal (loop bodies run: the escape step, or
+// max_iter for an orbit that never escapes), summed over all calls since the
+// last or_render_iterations().  The renderer kernels count the same quantity.
+std::atomic<uint64_t> g_render_iterations{0};
+
 // computeFractal<Real> of the two in-scope modules, bailout 4.
 template <class Real>
 float compute_fractal(int kind, uint32_t max_iter, Real px, Real py, double jcx, double jcy) {
@@ -76,9 +82,11 @@ float compute_fractal(int kind, uint32_t max_iter, Real px, Real py, double jcx,
     if (kind == CU_FRACTAL_JULIA) {
         // julia.cu:7 — c is the __constant__ double pair cast to Real; returns i.
         uint32_t i = escape_loop<Real>(max_iter, px, py, (Real)jcx, (Real)jcy, (Real)4, &m, &esc);
+        g_render_iterations.fetch_add(i, std::memory_order_relaxed);
         return (float)i;
     }
     uint32_t i = escape_loop<Real>(max_iter, (Real)0, (Real)0, px, py, (Real)4, &m, &esc);
+    g_render_iterations.fetch_add(i, std::memory_order_relaxed);
     if (i == max_iter) return 0;  // mandelbrot.cu:22-23
     return (float)i;
 }
@@ -525,6 +533,8 @@ uint64_t or_escape_tile_rows(const cu_tile_desc *t, int precision, uint32_t row0
     if (precision == 32) return tile_rows<float>(*t, row0, row1, counts, smooth, threads);
     return tile_rows<double>(*t, row0, row1, counts, smooth, threads);
 }
+
+uint64_t or_render_iterations(void) { return g_render_iterations.exchange(0); }
 
 void or_render_main(int precision, int kind, double jcx, double jcy, or_pixel_info *out,
                     int64_t pitch, uint32_t W, uint32_t H, double lbx, double lby, double rtx,

@@ -64,6 +64,9 @@ uint64_t or_escape_tile_rows(const cu_tile_desc *t, int precision, uint32_t row0
 /* fractalRenderMain<Real> (fractalRendererGeneric.cu:168-181) over a whole
  * image, with sampleTheFractal (:85-155) run in lock step per 8x4 vote group
  * (getImageIndexes, :36-53).  precision 32/64; kind CU_FRACTAL_*. */
+/* Iterations executed by computeFractal calls since the last call (render_main,
+ * render_advanced, render_undersampled, the point queries); resets the count. */
+uint64_t or_render_iterations(void);
 void or_render_main(int precision, int kind, double julia_cx, double julia_cy, or_pixel_info *out,
                     int64_t out_pitch_bytes, uint32_t width, uint32_t height, double lbx, double lby,
                     double rtx, double rty, uint32_t max_iter, float max_ss, uint32_t flags);

@@ -28,6 +28,7 @@ _SIG = {
     "or_tile_point_d": (None, [_P, _U32, _U32, _U32, C.POINTER(_D), C.POINTER(_D)]),
     "or_tile_point_f": (None, [_P, _U32, _U32, _U32, C.POINTER(_F), C.POINTER(_F)]),
     "or_escape_tile_rows": (_U64, [_P, _I, _U32, _U32, _P, _P, _I]),
+    "or_render_iterations": (C.c_uint64, []),
     "or_render_main": (None, [_I, _I, _D, _D, _P, C.c_int64, _U32, _U32, _D, _D, _D, _D, _U32, _F,
                               _U32]),
     "or_render_advanced": (None, [_I, _I, _D, _D, _P, C.c_int64, _U32, _U32, _D, _D, _D, _D, _U32,
@@ -102,6 +103,11 @@ def escape_tile(tile, precision: int, rows=None, threads: int = 0):
     it = lib().or_escape_tile_rows(C.byref(tile), precision, r0, r1, _ptr(counts), _ptr(smooth),
                                    threads)
     return counts[:, :w], smooth[:, :w], int(it)
+
+
+def render_iterations() -> int:
+    """Iterations executed by the oracle's computeFractal since the last call (resets)."""
+    return int(lib().or_render_iterations())
 
 
 def render_main(precision, kind, julia_c, w, h, seg, max_iter, max_ss, flags):

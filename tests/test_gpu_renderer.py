@@ -250,3 +250,41 @@ def test_driver_layer_under_sampled_kernel(gpu, oracle):
                         C.byref(C.c_double(1.0)), C.c_size_t(8))
     finally:
         cu.abi.check(L.cu_module_unload(mod))
+
+
+@pytest.mark.parametrize("adaptive", [True, False])
+def test_renderer_iteration_count_matches_oracle(gpu, provider, oracle, adaptive):
+    """cu_renderer_iterations: the module's executed-iteration total grows by
+    exactly the oracle's count for renderQuality and for a renderFast frame
+    (reuse + foveation), the figure bench.py turns into Giga-iterations/s."""
+    cu = gpu
+    r = provider.get_default_renderer()
+    w, h = 200, 136
+    pal = cu.default_palette()
+    r.initialize_rendering(w, h, pal)
+    try:
+        seg = cu.plane_segment_from_center(-0.7453, 0.1127, 0.02, w, h)
+        m = model_for(cu, seg, 900, 6.0, adaptive)
+        m.use_sample_reuse = 1
+        m.use_foveated_rendering = 1
+        flags = cu.FLAG_ADAPTIVE_SS if adaptive else 0
+        before = r.iterations()
+        r.render_quality(m)
+        prec = 32 if m.floating_point_precision == cu.PRECISION_SINGLE else 64
+        oracle.render_iterations()
+        prev = oracle.render_main(prec, cu.FRACTAL_MANDELBROT, (0, 0), w, h, seg, 900, 6.0, flags)
+        want = oracle.render_iterations()
+        assert want > 0 and r.iterations() - before == want
+        new = zoom_segment(seg, (50, 90), w, h, 0.977)
+        m.set_segment(*new)
+        m.zooming, m.zooming_in = 1, 1
+        m.mouse_focus_x, m.mouse_focus_y = 50, 90
+        before = r.iterations()
+        r.render_fast(m)
+        prec = 32 if m.floating_point_precision == cu.PRECISION_SINGLE else 64
+        oracle.render_advanced(prec, cu.FRACTAL_MANDELBROT, (0, 0), w, h, new, 900, 6.0,
+                               flags | cu.FLAG_SAMPLE_REUSE | cu.FLAG_ZOOMING | cu.FLAG_ZOOMING_IN |
+                               cu.FLAG_FOVEATION, seg, prev, (50, 90))
+        assert r.iterations() - before == oracle.render_iterations()
+    finally:
+        r.free_rendering_resources()
