@@ -21,10 +21,11 @@
 // re-renders glitched pixels around a secondary reference.
 //
 // Hot loop: unchecked blocks of K steps (one orbit load and 8 FMA-class
-// VALU ops per step); the rebase test is a per-step ballot OR-ed into a lane
-// mask, escape is tested once at the block end (escape is permanent for
-// bailout_sq >= 4, |c| <= 2), and a block with either event is replayed with
-// checked steps — the same operation sequence as the oracle's checked loop.
+// VALU ops per step).  A block ends early at the first step where some lane's
+// rebase test fires, and those lanes take the rebase there; escape is tested
+// once, at the block's end (escape is permanent for bailout_sq >= 4, |c| <= 2),
+// and only a block with an escape is replayed with checked steps — the same
+// operation sequence as the oracle's checked loop.
 //
 //   referenceOrbit{Float,Double}   one lane: double-double orbit of C -> entries, M
 //   perturb{Float,Double}          lane per pixel (8x8 pixels per wave, 32x32 per block,
@@ -63,20 +64,41 @@ __device__ __forceinline__ DD dd_mul(DD a, DD b) {
 }
 
 template <class Real>
-struct OrbitEntry {
+struct alignas(4 * sizeof(Real)) OrbitEntry {
     Real x, y, h, pad;  // Z_m and |Z_m|^2 / 2
 };
+
+// Pointers into LDS and global memory typed with their address spaces, so that
+// loads through them stay LDS or global (or scalar) loads: through untyped
+// pointers the compiler may merge reads from different branches into one
+// generic (flat) load of a selected address.
+template <class T>
+using lds_ptr = const T __attribute__((address_space(3))) *;
+template <class T>
+using global_ptr = const T __attribute__((address_space(1))) *;
+
+template <class Real, class Ptr>
+__device__ __forceinline__ OrbitEntry<Real> load_entry(Ptr p) {
+    OrbitEntry<Real> e;
+    e.x = p->x;
+    e.y = p->y;
+    e.h = p->h;
+    e.pad = 0;
+    return e;
+}
 
 // The reference orbit as the pixel kernels read it: entries below `staged`
 // from the block's LDS copy, the rest (an orbit longer than the LDS budget)
 // from global memory.
 template <class Real>
 struct OrbitView {
-    const OrbitEntry<Real> *global;
-    const OrbitEntry<Real> *lds;
+    global_ptr<OrbitEntry<Real>> global;
+    lds_ptr<OrbitEntry<Real>> lds;
     uint staged;
     __device__ __forceinline__ OrbitEntry<Real> at(uint m) const {
-        return m < staged ? lds[m] : global[m];
+        OrbitEntry<Real> e = load_entry<Real>(lds + (m < staged ? m : 0u));
+        if (m >= staged) e = load_entry<Real>(global + m);
+        return e;
     }
 };
 
@@ -186,15 +208,16 @@ __device__ __forceinline__ int checked_step(const OrbitView<Real> &Z, uint M, Re
     return 0;
 }
 
-// K unchecked steps from orbit entry zp[0]; returns the ballot mask of lanes
-// whose rebase test fired at any of the K states.
-template <class Real, int K>
-__device__ __forceinline__ uint64_t unchecked_block(const OrbitEntry<Real> *zp, Real &dx, Real &dy,
-                                                    Real dcx, Real dcy) {
+// K unchecked steps from orbit entry zp[0]; returns the ballot of lanes whose
+// rebase test fired at any of the K states.  No branches: the orbit loads of the
+// whole block can be issued ahead of the arithmetic.
+template <class Real, int K, class Ptr>
+__device__ __forceinline__ uint64_t unchecked_block(Ptr zp, Real &dx, Real &dy, Real dcx,
+                                                    Real dcy) {
     uint64_t flagged = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const OrbitEntry<Real> e = zp[k];
+        const OrbitEntry<Real> e = load_entry<Real>(zp + k);
         const Real t = cu::fma_r(e.x, dx, cu::fma_r(e.y, dy, e.h));
         flagged |= cu::ballot(t < (Real)0);
         const Real ax = cu::fma_r((Real)2, e.x, dx);
@@ -207,11 +230,49 @@ __device__ __forceinline__ uint64_t unchecked_block(const OrbitEntry<Real> *zp, 
     return flagged;
 }
 
+// The same steps, stopping at the first step k0 at which some lane's rebase test
+// fires: every lane is left in its state at step k0, `fired` holds the lanes whose
+// test fired there and `end` is orbit entry zp[k0].  Returns k0.  Only run after
+// unchecked_block reported a firing in the block, so k0 < K.
+template <class Real, int K, class Ptr>
+__device__ __forceinline__ uint stopping_block(Ptr zp, Real &dx, Real &dy, Real dcx, Real dcy,
+                                               uint64_t &fired, OrbitEntry<Real> &end) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const OrbitEntry<Real> e = load_entry<Real>(zp + k);
+        const Real t = cu::fma_r(e.x, dx, cu::fma_r(e.y, dy, e.h));
+        fired = cu::ballot(t < (Real)0);
+        if (fired != 0) {
+            end = e;
+            return (uint)k;
+        }
+        const Real ax = cu::fma_r((Real)2, e.x, dx);
+        const Real ay = cu::fma_r((Real)2, e.y, dy);
+        const Real ndx = cu::fma_r(ax, dx, cu::fma_r(-ay, dy, dcx));
+        const Real ndy = cu::fma_r(ax, dy, cu::fma_r(ay, dx, dcy));
+        dx = ndx;
+        dy = ndy;
+    }
+    end = load_entry<Real>(zp + K);
+    return (uint)K;
+}
+
 // Whole pixel; lanes whose `active` is false only take part in the ballots.
 // When every lane running a block reads the same orbit entries (same m), the
 // block reads them once per wave through the scalar cache; otherwise (lanes
 // rebased at different steps) each lane reads its own entries from the LDS
 // copy, or from global memory past it.
+//
+// A block ends at step k0 for the whole wave: K, or the first step at which a
+// lane's rebase test fired.  There the escape test runs on z = Z_m + d.  Escaped
+// (at step k0, hence at some step of the block since escape is permanent): the
+// block is replayed with checked steps from its start.  Otherwise the lanes whose
+// test fired take the rebase d = z, m = 0 with n counting the k0 steps taken, and
+// the others simply stand at step k0.  The checked loop would take the same
+// rebase at the same state and then step from Z_0 = 0; the next block starts
+// there, and its first rebase test, at m = 0 with Z_0 = |Z_0|^2 = 0, gives t = 0
+// and does not fire, while lanes that did not fire repeat a test that passed —
+// so every block advances.
 template <class Real, int K>
 __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dcy, uint max_iter,
                              Real bsq, bool rebase, Real geps, bool active, uint &n_out,
@@ -227,31 +288,49 @@ __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dc
         bool replay = false;
         if (blk) {
             const Real sx = dx, sy = dy;
-            const uint sm = m, sn = n;
             const uint mu = __builtin_amdgcn_readfirstlane(m);
+            // where this block reads the orbit: the same entries for the whole wave
+            // (scalar loads), or per lane from LDS, or per lane past the LDS copy
+            const int where = cu::ballot(m != mu) == 0 ? 0 : (m + (uint)K < Z.staged ? 1 : 2);
             uint64_t flagged;
             OrbitEntry<Real> e;
-            if (cu::ballot(m != mu) == 0) {
+            if (where == 0) {
+                e = load_entry<Real>(Z.global + (mu + (uint)K));
                 flagged = unchecked_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy);
-                e = Z.global[mu + (uint)K];
+            } else if (where == 1) {
+                e = load_entry<Real>(Z.lds + (m + (uint)K));
+                flagged = unchecked_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy);
             } else {
-                if (m + (uint)K <= Z.staged)
-                    flagged = unchecked_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy);
-                else
-                    flagged = unchecked_block<Real, K>(Z.global + m, dx, dy, dcx, dcy);
-                e = Z.at(m + (uint)K);
+                e = load_entry<Real>(Z.global + (m + (uint)K));
+                flagged = unchecked_block<Real, K>(Z.global + m, dx, dy, dcx, dcy);
             }
-            m += K;
-            n += K;
+            uint k0 = K;
+            uint64_t fired = 0;
+            if (flagged != 0) {  // a rebase test fired: rerun the block up to that step
+                dx = sx;
+                dy = sy;
+                if (where == 0)
+                    k0 = stopping_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy, fired, e);
+                else if (where == 1)
+                    k0 = stopping_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy, fired, e);
+                else
+                    k0 = stopping_block<Real, K>(Z.global + m, dx, dy, dcx, dcy, fired, e);
+            }
             const Real zx = e.x + dx;
             const Real zy = e.y + dy;
             const Real mag = cu::fma_r(zx, zx, zy * zy);
-            if (((flagged >> lane) & 1ull) || !(mag < bsq)) {
+            if (!(mag < bsq)) {
                 dx = sx;
                 dy = sy;
-                m = sm;
-                n = sn;
                 replay = true;
+            } else {
+                n += k0;
+                m += k0;
+                if ((fired >> lane) & 1ull) {
+                    dx = zx;
+                    dy = zy;
+                    m = 0;
+                }
             }
         }
         if (state == 0 && (!blk || replay)) {
@@ -297,7 +376,8 @@ __device__ void perturb_frame(const DeepArgs a, const OrbitEntry<Real> *__restri
     const uint staged = min(M + 1u, kOrbitLdsBytes / (uint)sizeof(OrbitEntry<Real>));
     for (uint k = threadIdx.x; k < staged; k += blockDim.x) lds[k] = orbit[k];
     __syncthreads();
-    const OrbitView<Real> view{orbit, lds, staged};
+    const OrbitView<Real> view{(global_ptr<OrbitEntry<Real>>)orbit,
+                               (lds_ptr<OrbitEntry<Real>>)lds, staged};
     uint px, py;
     deep_pixel_coords<4>(px, py);
     const uint64_t o = (uint64_t)py * a.width + px;

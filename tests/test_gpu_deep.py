@@ -55,11 +55,13 @@ def test_orbit_bit_exact(gpu, engine, method, precision):
     assert orbit.shape == want.shape and np.array_equal(orbit, want)
 
 
-@pytest.mark.parametrize("width", [1e-6, 1e-10, 1e-20])
+# 1.0: a shallow frame around one reference, where rebases come every few steps
+# and most blocks stop early at a rebase
+@pytest.mark.parametrize("width", [1e-6, 1e-10, 1e-20, 1.0])
 @pytest.mark.parametrize("method", [32, 64])
 def test_perturbation_matches_oracle(gpu, engine, width, method):
     cu = gpu
-    d = deep(cu, width, max_iter=50000 if width != 1e-6 else 20000)
+    d = deep(cu, width, max_iter={1e-6: 20000, 1.0: 2000}.get(width, 50000))
     counts, smooth, glitch, rep, orbit = render(cu, engine, d, method)
     ys, xs = np.mgrid[0:d.height, 0:d.width]
     om = O.DEEP_PERTURB_F32 if method == 32 else O.DEEP_PERTURB_F64
