@@ -432,14 +432,17 @@ extern "C" __global__ __launch_bounds__(64) void referenceOrbitDouble(
 }
 // `orbit` repeats a.orbit as its own read-only argument, so uniform loads from
 // it can go through the scalar cache.  1024 threads: 32 x 32 pixels per block
-// share one LDS copy of the orbit.
-extern "C" __global__ __launch_bounds__(1024) void perturbFloat(
+// share one LDS copy of the orbit.  The second bound (8 waves per SIMD) keeps
+// the kernels within 64 VGPRs, so that two blocks (their 2 x 64 KB of LDS) run
+// on each CU; without it the compiler keeps a whole 16-step block's orbit reads
+// in flight and only one block fits.
+extern "C" __global__ __launch_bounds__(1024, 8) void perturbFloat(
     DeepArgs a, const OrbitEntry<float> *__restrict__ orbit) {
-    perturb_frame<float, 8>(a, orbit);
+    perturb_frame<float, 16>(a, orbit);
 }
-extern "C" __global__ __launch_bounds__(1024) void perturbDouble(
+extern "C" __global__ __launch_bounds__(1024, 8) void perturbDouble(
     DeepArgs a, const OrbitEntry<double> *__restrict__ orbit) {
-    perturb_frame<double, 8>(a, orbit);
+    perturb_frame<double, 16>(a, orbit);
 }
 extern "C" __global__ __launch_bounds__(256) void deepDirectDoubleExact(DeepArgs a) {
     deep_direct<16, false>(a);
