@@ -317,5 +317,10 @@ __device__ __forceinline__ void tile_point<float>(const cu_tile_desc &t, uint px
 // Wave-level helpers (wave64).
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ uint lane_id() { return __lane_id(); }
+// This lane's bit of a wave-uniform lane mask (the inverse of ballot): the mask
+// becomes the execution mask itself, with no per-lane shift and compare.
+__device__ __forceinline__ bool in_mask(uint64_t mask) {
+    return __builtin_amdgcn_inverse_ballot_w64(mask);
+}
 
 }  // namespace cu

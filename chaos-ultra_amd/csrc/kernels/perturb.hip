@@ -232,16 +232,18 @@ __device__ __forceinline__ uint64_t unchecked_block(Ptr zp, Real &dx, Real &dy, 
 
 // The same steps, stopping at the first step k0 at which some lane's rebase test
 // fires: every lane is left in its state at step k0, `fired` holds the lanes whose
-// test fired there and `end` is orbit entry zp[k0].  Returns k0.  Only run after
-// unchecked_block reported a firing in the block, so k0 < K.
+// test fired there and `end` is orbit entry zp[k0].  Returns k0.  Only lanes in
+// `running` count.  Only run after unchecked_block reported a firing of a
+// running lane in the block, so k0 < K.
 template <class Real, int K, class Ptr>
 __device__ __forceinline__ uint stopping_block(Ptr zp, Real &dx, Real &dy, Real dcx, Real dcy,
-                                               uint64_t &fired, OrbitEntry<Real> &end) {
+                                               uint64_t running, uint64_t &fired,
+                                               OrbitEntry<Real> &end) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const OrbitEntry<Real> e = load_entry<Real>(zp + k);
         const Real t = cu::fma_r(e.x, dx, cu::fma_r(e.y, dy, e.h));
-        fired = cu::ballot(t < (Real)0);
+        fired = cu::ballot(t < (Real)0) & running;
         if (fired != 0) {
             end = e;
             return (uint)k;
@@ -258,21 +260,26 @@ __device__ __forceinline__ uint stopping_block(Ptr zp, Real &dx, Real &dy, Real 
 }
 
 // Whole pixel; lanes whose `active` is false only take part in the ballots.
-// When every lane running a block reads the same orbit entries (same m), the
-// block reads them once per wave through the scalar cache; otherwise (lanes
-// rebased at different steps) each lane reads its own entries from the LDS
-// copy, or from global memory past it.
+//
+// While every running lane can take an unchecked block (its iteration budget and
+// the orbit both have K steps left), the whole wave takes one, in wave-uniform
+// control flow: lanes that have finished compute along with their m held at 0,
+// so every read stays in bounds, and change nothing (ballots are masked to the
+// running lanes).  When the running lanes all read the same orbit entries (same
+// m), the block reads them through the scalar cache; otherwise each lane reads
+// its own entries from the LDS copy, or from global memory past it.  When some
+// running lane cannot take a block, every running lane takes one checked step.
 //
 // A block ends at step k0 for the whole wave: K, or the first step at which a
-// lane's rebase test fired.  There the escape test runs on z = Z_m + d.  Escaped
-// (at step k0, hence at some step of the block since escape is permanent): the
-// block is replayed with checked steps from its start.  Otherwise the lanes whose
-// test fired take the rebase d = z, m = 0 with n counting the k0 steps taken, and
-// the others simply stand at step k0.  The checked loop would take the same
-// rebase at the same state and then step from Z_0 = 0; the next block starts
-// there, and its first rebase test, at m = 0 with Z_0 = |Z_0|^2 = 0, gives t = 0
-// and does not fire, while lanes that did not fire repeat a test that passed —
-// so every block advances.
+// running lane's rebase test fired.  There the escape test runs on z = Z_m + d.
+// Escaped (at step k0, hence at some step of the block since escape is
+// permanent): the lane replays the block with checked steps from its start.
+// Otherwise the lanes whose test fired take the rebase d = z, m = 0 with n
+// counting the k0 steps taken, and the others simply stand at step k0.  The
+// checked loop would take the same rebase at the same state and then step from
+// Z_0 = 0; the next block starts there, and its first rebase test, at m = 0 with
+// Z_0 = |Z_0|^2 = 0, gives t = 0 and does not fire, while lanes that did not
+// fire repeat a test that passed — so every block advances.
 template <class Real, int K>
 __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dcy, uint max_iter,
                              Real bsq, bool rebase, Real geps, bool active, uint &n_out,
@@ -281,23 +288,23 @@ __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dc
     uint m = 0, n = 0;
     int state = active ? 0 : 4;
     mag_out = 0;
-    const uint lane = cu::lane_id();
     for (;;) {
-        if (cu::ballot(state == 0) == 0) break;
-        const bool blk = state == 0 && rebase && max_iter - n >= (uint)K && M - m >= (uint)K;
+        const uint64_t running = cu::ballot(state == 0);
+        if (running == 0) break;
+        const bool can = rebase && max_iter - n >= (uint)K && M - m >= (uint)K;
+        const bool block = (cu::ballot(can) & running) == running;
         bool replay = false;
-        if (blk) {
+        if (block) {
             const Real sx = dx, sy = dy;
-            const uint mu = __builtin_amdgcn_readfirstlane(m);
-            // where this block reads the orbit: the same entries for the whole wave
-            // (scalar loads), or per lane from LDS, or per lane past the LDS copy
-            const int where = cu::ballot(m != mu) == 0 ? 0 : (m + (uint)K < Z.staged ? 1 : 2);
+            const uint mu = __builtin_amdgcn_readlane(m, (int)__builtin_ctzll(running));
+            const bool same = (cu::ballot(m != mu) & running) == 0;
+            const bool lds = (cu::ballot(m + (uint)K >= Z.staged) & running) == 0;
             uint64_t flagged;
             OrbitEntry<Real> e;
-            if (where == 0) {
+            if (same) {
                 e = load_entry<Real>(Z.global + (mu + (uint)K));
                 flagged = unchecked_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy);
-            } else if (where == 1) {
+            } else if (lds) {
                 e = load_entry<Real>(Z.lds + (m + (uint)K));
                 flagged = unchecked_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy);
             } else {
@@ -306,38 +313,42 @@ __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dc
             }
             uint k0 = K;
             uint64_t fired = 0;
-            if (flagged != 0) {  // a rebase test fired: rerun the block up to that step
+            if ((flagged & running) != 0) {  // a rebase test fired: rerun the block up to there
                 dx = sx;
                 dy = sy;
-                if (where == 0)
-                    k0 = stopping_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy, fired, e);
-                else if (where == 1)
-                    k0 = stopping_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy, fired, e);
+                if (same)
+                    k0 = stopping_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy, running, fired, e);
+                else if (lds)
+                    k0 = stopping_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy, running, fired, e);
                 else
-                    k0 = stopping_block<Real, K>(Z.global + m, dx, dy, dcx, dcy, fired, e);
+                    k0 = stopping_block<Real, K>(Z.global + m, dx, dy, dcx, dcy, running, fired,
+                                                 e);
             }
-            const Real zx = e.x + dx;
-            const Real zy = e.y + dy;
-            const Real mag = cu::fma_r(zx, zx, zy * zy);
-            if (!(mag < bsq)) {
-                dx = sx;
-                dy = sy;
-                replay = true;
-            } else {
-                n += k0;
-                m += k0;
-                if ((fired >> lane) & 1ull) {
-                    dx = zx;
-                    dy = zy;
-                    m = 0;
+            if (state == 0) {
+                const Real zx = e.x + dx;
+                const Real zy = e.y + dy;
+                const Real mag = cu::fma_r(zx, zx, zy * zy);
+                if (!(mag < bsq)) {
+                    dx = sx;
+                    dy = sy;
+                    replay = true;
+                } else {
+                    n += k0;
+                    m += k0;
+                    if (cu::in_mask(fired)) {
+                        dx = zx;
+                        dy = zy;
+                        m = 0;
+                    }
                 }
             }
         }
-        if (state == 0 && (!blk || replay)) {
-            const int steps = blk ? K : 1;
+        if (state == 0 && (!block || replay)) {
+            const int steps = block ? K : 1;
             for (int s = 0; s < steps && state == 0; ++s)
                 state = checked_step<Real>(Z, M, dx, dy, m, n, dcx, dcy, max_iter, bsq, rebase,
                                            geps, mag_out);
+            if (state != 0) m = 0;  // finished: later blocks read from entry 0
         }
     }
     n_out = n;
