@@ -95,6 +95,7 @@ __device__ __forceinline__ bool run_lanes(const PipeArgs &a, bool active, Real x
                                           double &smooth) {
     // mode: 0 none, 1 running blocks, 2 pending (checked finish)
     int mode = !active ? 0 : ((blocks && max_iter - i >= (uint)K) ? 1 : 2);
+    const uint i0 = i;  // iterations the sample had executed before this pass
     bool replay = false;
     Real sx = x, sy = y, mx = x, my = y;
     Real x2 = x * x, y2 = y * y;          // squares of the current state, carried
@@ -128,14 +129,13 @@ __device__ __forceinline__ bool run_lanes(const PipeArgs &a, bool active, Real x
         if (!keep_going()) break;
         block();
     }
-    uint64_t executed = 0;
     count = 0;
     smooth = 0;
-    if (mode == 2) {
+    if (mode == 2)
         finish_pending<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, max_iter, bsq, smooth_c,
                                         julia, replay, count, smooth);
-        executed = i;
-    }
+    // this pass's own iterations, whether the sample finished or is handed on
+    const uint64_t executed = mode != 0 ? (uint64_t)(i - i0) : 0u;
     // lanes still running: append to this wave's sub-queue (one atomic per wave)
     const bool push = mode == 1;
     const uint64_t pushers = cu::ballot(push);
@@ -375,7 +375,7 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
     int mode = 0;  // 0 idle, 1 running unchecked blocks, 2 pending
     bool replay = false;
     Real x = 0, y = 0, cx = 0, cy = 0, sx = 0, sy = 0, mx = 0, my = 0, bsq = 4;
-    uint i = 0, tile = 0, sample = 0, max_iter = 0;
+    uint i = 0, i0 = 0, tile = 0, sample = 0, max_iter = 0;
     double smooth_c = 1;
     bool julia = false, blocks = false;
     uint64_t executed = 0;
@@ -406,6 +406,7 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
                 const Straggler<Real> e = queue[(uint64_t)cq * a.subq_capacity + cur + rank];
                 load_straggler<Real>(a, e, x, y, cx, cy, i, tile, sample, max_iter, bsq, smooth_c,
                                      julia, blocks);
+                i0 = i;
                 replay = false;
                 mode = (blocks && max_iter - i >= (uint)K) ? 1 : 2;
             }
@@ -445,7 +446,7 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
             double sm;
             finish_pending<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, max_iter, bsq,
                                             smooth_c, julia, replay, count, sm);
-            executed += i;
+            executed += i - i0;  // the iterations run here; earlier passes counted theirs
             store_sample(a, tile, sample, count, sm);
             mode = 0;
         }

@@ -54,6 +54,29 @@ def test_c2_frame_pipeline_vs_oracle(gpu, engine, oracle, tune, precision):
     assert_smooth_close(smooth.reshape(384, 512), want_s)
 
 
+def test_pass1_iterations_are_its_own(gpu, engine, oracle):
+    """cu_frame_iterations' pass-1 figure counts the iterations pass 1 itself ran,
+    including the steps of the samples it hands on, and the tail passes count only
+    their own steps: the two add up to the oracle's total, and the later pass 1
+    hands its stragglers on (1 lane left instead of 24), the more of the work it
+    has done."""
+    cu = gpu
+    seg = cu.plane_segment_from_center(*C2_CENTER, 1e-4 / 8, 512, 384)
+    tile = cu.make_tile(*seg, 512, 384, 5000, 256.0, spp=4, pattern=cu.SAMPLE_JITTER, seed=3)
+    _, _, want_it = oracle.escape_tile(tile, 64)
+    f = cu.Frame(engine, [tile], 64, -1)
+    try:
+        first = {}
+        for handoff in (1, 24):
+            f.render(cu.FrameTune(16, handoff, 3, 0, 0))
+            cu.synchronize()
+            assert f.iterations() == want_it
+            first[handoff] = f.iterations_pass1()
+    finally:
+        f.close()
+    assert 0.5 * want_it < first[24] < first[1] < want_it
+
+
 def test_c1_frame_pipeline_matches_reference(gpu, engine):
     cu = gpu
     with open(os.path.join(GOLDEN, "reference_iteration.json")) as f:
