@@ -13,7 +13,9 @@ Also reported (variants): the 256-tile uneven batch (autotuned, captured
 replay), 16 zoom sessions through the continuous-batching loop (p50/p99
 arrival-to-completion latency), the drop-in FractalRenderer path at 3840x2160
 (renderQuality and renderFast), the 1920x1080 deep-zoom frames (FP32 and FP64-delta perturbation and
-direct FP64 with their mismatch against the double-double oracle subset), and
+direct FP64 with their mismatch against the double-double oracle subset and their
+algorithmic FLOP rate against the FP32 / FP64 vector peak: 16 FLOPs per
+perturbation step, 8 per direct step, whole frame including the reference orbit), and
 the roofline of the dominant kernel (pass 1 of the pipeline:
 8 algorithmic FP64 FLOPs per iteration it executes, against the 78.6 TFLOP/s
 FP64 vector peak and the 39.3 TFLOP/s add/mul-only ceiling), the thread-per-
@@ -46,6 +48,9 @@ MAX_ITER = 5000
 BAILOUT_SQ = 256.0
 SEED = 20261202
 FLOPS_PER_ITER = 8
+# a perturbation step: (2Z + d) d + dc as 6 FMA-class operations and the rebase
+# test fma(Zx, dx, fma(Zy, dy, |Z|^2/2)), 8 FMAs = 16 FLOPs
+PERTURB_FLOPS_PER_ITER = 16
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 TUNE = (16, 24, 3)  # block_steps, handoff_lanes, tail_passes
@@ -214,7 +219,11 @@ def deep_leg(cu, engine):
             counts = f.counts().ravel()[idx]
             f.close()
             m = float(np.median(ms))
+            flops = PERTURB_FLOPS_PER_ITER if method != cu.DEEP_DIRECT_F64 else FLOPS_PER_ITER
+            peak = FP32_PEAK_TFLOPS if method == cu.DEEP_PERTURB_F32 else FP64_PEAK_TFLOPS
+            tflops = rep.iterations * flops / (m * 1e-3) / 1e12
             row[name] = {"frame_ms": round(m, 3), "giga_iter_s": round(rep.iterations / m / 1e6, 1),
+                         "tflops": round(tflops, 2), "frac_of_vector_peak": round(tflops / peak, 4),
                          "mismatch_vs_dd_pct": round(100 * float(np.mean(counts != g[f"counts_{w:.0e}"])), 3)}
             if method != cu.DEEP_DIRECT_F64:
                 row[name]["orbit_ms"] = round(rep.orbit_ms, 3)

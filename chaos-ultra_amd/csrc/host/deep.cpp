@@ -169,11 +169,12 @@ void launch_pixels(cu_deep d, cu_stream s, DeepArgs &a) {
     cu_function f = function(d->engine, name);
     const void *orbit = a.orbit;
     void *params[] = {&a, &orbit};  // direct kernels take only the first
-    // direct: 256 threads = 16 x 16 pixels; perturbation: 1024 threads = 32 x 32
-    // pixels sharing one LDS copy of the reference orbit
-    const uint32_t side = d->method == CU_DEEP_DIRECT_F64 ? 16 : 32;
-    dim3 grid((d->desc.width + side - 1) / side, (d->desc.height + side - 1) / side, 1);
-    launch(d->engine, f, grid, dim3(side * side), params, s, false);
+    // direct and FP32 perturbation: 256 threads = 16 x 16 pixels; FP64
+    // perturbation: 512 threads = 32 x 16 pixels (a perturbation block shares
+    // one LDS copy of the reference orbit)
+    const uint32_t bw = d->method == CU_DEEP_PERTURB_F64 ? 32 : 16, bh = 16;
+    dim3 grid((d->desc.width + bw - 1) / bw, (d->desc.height + bh - 1) / bh, 1);
+    launch(d->engine, f, grid, dim3(bw * bh), params, s, false);
 }
 
 // The glitched pixel nearest the centroid of all glitched pixels.

@@ -28,7 +28,7 @@
 // operation sequence as the oracle's checked loop.
 //
 //   referenceOrbit{Float,Double}   one lane: double-double orbit of C -> entries, M
-//   perturb{Float,Double}          lane per pixel (8x8 pixels per wave, 32x32 per block,
+//   perturb{Float,Double}          lane per pixel (8x8 pixels per wave, 16x16 / 32x16 per block,
 //                                  orbit staged in LDS)
 //   deepDirectDouble               direct FP64 at the same pixel -> c mapping
 
@@ -368,29 +368,31 @@ __device__ __forceinline__ void deep_write(const DeepArgs &a, uint64_t o, int st
     }
 }
 
-// A block of W x W waves, each wave 8 x 8 pixels (256 threads: W = 2; 1024: W = 4).
-template <uint W>
+// A block of WX x WY waves, each wave 8 x 8 pixels.
+template <uint WX, uint WY>
 __device__ __forceinline__ void deep_pixel_coords(uint &px, uint &py) {
     const uint lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    px = (blockIdx.x * W + (wave % W)) * 8u + (lane & 7u);
-    py = (blockIdx.y * W + (wave / W)) * 8u + (lane >> 3);
+    px = (blockIdx.x * WX + (wave % WX)) * 8u + (lane & 7u);
+    py = (blockIdx.y * WY + (wave / WX)) * 8u + (lane >> 3);
 }
 
-// Bytes of reference orbit each perturbation block stages in LDS (4096 FP32 /
-// 2048 FP64 entries); two 1024-thread blocks share a CU's 160 KB.
-constexpr uint kOrbitLdsBytes = 64u * 1024u;
-
-template <class Real, int K>
+// Perturbation blocks stage the first LdsBytes of the orbit in LDS and share it
+// among their WX x WY waves.  A CU holds 160 KB of LDS and 32 waves, and a
+// block's wave slots only free up when its slowest wave ends, so smaller blocks
+// keep more waves resident; larger copies keep more of the orbit in LDS.
+//   FP32: 2 x 2 waves (16 x 16 pixels, 256 threads), 20 KB = 1280 entries, 8 blocks per CU
+//   FP64: 4 x 2 waves (32 x 16 pixels, 512 threads), 40 KB = 1280 entries, 4 blocks per CU
+template <class Real, int K, uint WX, uint WY, uint LdsBytes>
 __device__ void perturb_frame(const DeepArgs a, const OrbitEntry<Real> *__restrict__ orbit) {
-    __shared__ OrbitEntry<Real> lds[kOrbitLdsBytes / sizeof(OrbitEntry<Real>)];
+    __shared__ OrbitEntry<Real> lds[LdsBytes / sizeof(OrbitEntry<Real>)];
     const uint M = *a.orbit_len;
-    const uint staged = min(M + 1u, kOrbitLdsBytes / (uint)sizeof(OrbitEntry<Real>));
+    const uint staged = min(M + 1u, LdsBytes / (uint)sizeof(OrbitEntry<Real>));
     for (uint k = threadIdx.x; k < staged; k += blockDim.x) lds[k] = orbit[k];
     __syncthreads();
     const OrbitView<Real> view{(global_ptr<OrbitEntry<Real>>)orbit,
                                (lds_ptr<OrbitEntry<Real>>)lds, staged};
     uint px, py;
-    deep_pixel_coords<4>(px, py);
+    deep_pixel_coords<WX, WY>(px, py);
     const uint64_t o = (uint64_t)py * a.width + px;
     bool active = px < a.width && py < a.height;
     if (active && a.only_glitched) active = a.glitch[o] != 0;
@@ -414,7 +416,7 @@ __device__ void perturb_frame(const DeepArgs a, const OrbitEntry<Real> *__restri
 template <int K, bool kFused>
 __device__ void deep_direct(const DeepArgs a) {
     uint px, py;
-    deep_pixel_coords<2>(px, py);
+    deep_pixel_coords<2, 2>(px, py);
     const bool active = px < a.width && py < a.height;
     uint64_t executed = 0;
     if (active) {
@@ -442,18 +444,17 @@ extern "C" __global__ __launch_bounds__(64) void referenceOrbitDouble(
     reference_orbit<double>(cx_hi, cx_lo, cy_hi, cy_lo, max_iter, bailout_sq, orbit, length);
 }
 // `orbit` repeats a.orbit as its own read-only argument, so uniform loads from
-// it can go through the scalar cache.  1024 threads: 32 x 32 pixels per block
-// share one LDS copy of the orbit.  The second bound (8 waves per SIMD) keeps
-// the kernels within 64 VGPRs, so that two blocks (their 2 x 64 KB of LDS) run
-// on each CU; without it the compiler keeps a whole 16-step block's orbit reads
-// in flight and only one block fits.
-extern "C" __global__ __launch_bounds__(1024, 8) void perturbFloat(
+// it can go through the scalar cache.  The second bound (8 waves per SIMD)
+// keeps the kernels within 64 VGPRs, so that a CU's 32 wave slots can all be
+// filled; without it the compiler keeps a whole 16-step block's orbit reads in
+// flight.  Block shapes: perturb_frame.
+extern "C" __global__ __launch_bounds__(256, 8) void perturbFloat(
     DeepArgs a, const OrbitEntry<float> *__restrict__ orbit) {
-    perturb_frame<float, 16>(a, orbit);
+    perturb_frame<float, 16, 2, 2, 20u * 1024u>(a, orbit);
 }
-extern "C" __global__ __launch_bounds__(1024, 8) void perturbDouble(
+extern "C" __global__ __launch_bounds__(512, 8) void perturbDouble(
     DeepArgs a, const OrbitEntry<double> *__restrict__ orbit) {
-    perturb_frame<double, 16>(a, orbit);
+    perturb_frame<double, 16, 4, 2, 40u * 1024u>(a, orbit);
 }
 extern "C" __global__ __launch_bounds__(256) void deepDirectDoubleExact(DeepArgs a) {
     deep_direct<16, false>(a);
