@@ -28,6 +28,7 @@ python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-variants]
 from __future__ import annotations
 
 import argparse
+import ctypes
 import importlib
 import json
 import os
@@ -388,6 +389,44 @@ def main():
     frame_tflops = iters * FLOPS_PER_ITER / (float(np.mean(render_ms)) * 1e-3) / 1e12
     log(f"[bench] FP64 {value:.1f} G-iter/s, {ms_step:.3f} ms/step, pass 1 {pass1_ms:.3f} ms "
         f"({iters1 / iters:.3f} of the iterations), render p50 {np.median(render_ms):.3f} ms")
+    # end to end (outside the timed steps): a render, then the RGBA8 image copied to
+    # ordinary host memory, as a caller reading the frame into its own array would
+    host = np.empty(W * H, dtype=np.uint32)
+    e2e_ms, copy_ms = [], []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run.step()
+        cu.synchronize()
+        t1 = time.perf_counter()
+        cu.abi.call("cu_memcpy_dtoh", host.ctypes.data_as(ctypes.c_void_p), run.rgba.ptr,
+                    ctypes.c_size_t(host.nbytes))
+        t2 = time.perf_counter()
+        e2e_ms.append((t2 - t0) * 1e3)
+        copy_ms.append((t2 - t1) * 1e3)
+    # the same into page-locked host memory (cu_host_alloc)
+    pinned = cu.HostBuffer(host.nbytes)
+    e2e_pin, copy_pin = [], []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run.step()
+        cu.synchronize()
+        t1 = time.perf_counter()
+        pinned.download_from(run.rgba)
+        t2 = time.perf_counter()
+        e2e_pin.append((t2 - t0) * 1e3)
+        copy_pin.append((t2 - t1) * 1e3)
+    pinned.close()
+
+    def readback_of(e2e, cp):
+        return {"frame_ms_p50": round(float(np.median(e2e)), 3),
+                "copy_ms_p50": round(float(np.median(cp)), 3),
+                "copy_gb_s": round(host.nbytes / (float(np.median(cp)) * 1e-3) / 1e9, 1),
+                "giga_iter_s": round(iters / (float(np.median(e2e)) * 1e-3) / 1e9, 1)}
+    readback = {"pageable": readback_of(e2e_ms, copy_ms), "page_locked": readback_of(e2e_pin, copy_pin),
+                "note": "render + device-to-host copy of the 64 MiB RGBA8 image into ordinary "
+                        "(pageable) or cu_host_alloc (page-locked) host memory; not part of value "
+                        "(the image stays in HBM)"}
+    log(f"[bench] with read-back: {readback}")
     run.close()
 
     variants = {}
@@ -443,7 +482,8 @@ def main():
                             "tail_passes": TUNE[2]},
                    "iterations_per_step": int(iters),
                    "render_ms_p50": round(float(np.median(render_ms)), 3),
-                   "render_ms_p99": round(float(np.percentile(render_ms, 99)), 3)},
+                   "render_ms_p99": round(float(np.percentile(render_ms, 99)), 3),
+                   "with_readback": readback},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": read_traffic(),

@@ -96,6 +96,38 @@ class DeviceBuffer:
             pass
 
 
+class HostBuffer:
+    """Page-locked host memory (cu_host_alloc) freed on close()/garbage collection;
+    array() views it as a numpy array, download_from() copies a device buffer in."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        abi.call("cu_host_alloc", C.byref(self.ptr), C.c_size_t(self.nbytes))
+
+    def array(self, dtype) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        raw = (C.c_ubyte * self.nbytes).from_address(self.ptr.value)
+        return np.frombuffer(raw, dtype=dtype, count=self.nbytes // dtype.itemsize)
+
+    def download_from(self, dev: DeviceBuffer, nbytes: int | None = None) -> None:
+        n = min(self.nbytes, dev.nbytes) if nbytes is None else int(nbytes)
+        if n > self.nbytes or n > dev.nbytes:
+            raise ValueError("copy larger than a buffer")
+        abi.call("cu_memcpy_dtoh", self.ptr, dev.ptr, C.c_size_t(n))
+
+    def close(self) -> None:
+        if self.ptr and self.ptr.value:
+            abi.call("cu_host_free", self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class IterationCounter(DeviceBuffer):
     """CU_ITER_SLOTS device u64 slots receiving the iterations executed by a launch."""
 

@@ -208,6 +208,8 @@ SIGNATURES = {
     "cu_mem_alloc": (_I, [C.POINTER(_P), _SZ]),
     "cu_mem_alloc_pitch": (_I, [C.POINTER(_P), C.POINTER(_SZ), _SZ, _SZ, C.c_uint]),
     "cu_mem_free": (_I, [_P]),
+    "cu_host_alloc": (_I, [C.POINTER(_P), _SZ]),
+    "cu_host_free": (_I, [_P]),
     "cu_memcpy_htod": (_I, [_P, _P, _SZ]),
     "cu_memcpy_dtoh": (_I, [_P, _P, _SZ]),
     "cu_memcpy_dtod": (_I, [_P, _P, _SZ]),

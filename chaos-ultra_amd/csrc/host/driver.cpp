@@ -204,6 +204,24 @@ int cu_mem_free(void *dptr) {
     });
 }
 
+int cu_host_alloc(void **ptr, size_t bytes) {
+    return guarded([&] {
+        if (!ptr) fail(CU_ERR_ILLEGAL_ARGUMENT, "ptr is NULL");
+        *ptr = nullptr;
+        if (!bytes) fail(CU_ERR_ILLEGAL_ARGUMENT, "bytes must be > 0");
+        ensure_device();
+        check(hipHostMalloc(ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+    });
+}
+
+int cu_host_free(void *ptr) {
+    return guarded([&] {
+        if (!ptr) return;
+        ensure_device();
+        check(hipHostFree(ptr), "hipHostFree");
+    });
+}
+
 int cu_memcpy_htod(void *dst, const void *src, size_t bytes) {
     return guarded([&] {
         if (!bytes) return;

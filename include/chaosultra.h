@@ -89,6 +89,12 @@ int cu_memcpy_2d_dtoh(void *dst, size_t dst_pitch, const void *src, size_t src_p
                       size_t width_bytes, size_t height);
 /* Synchronous fill of `count` 32-bit words. */
 int cu_memset_u32(void *dptr, uint32_t value, size_t count);
+/* Page-locked host memory (hipHostMalloc) for reading frames back: device-to-host
+ * copies into it run at several times the rate of copies into pageable memory.
+ * No counterpart in the reference, whose frames reach the host through GL
+ * interop; for callers that read frames back, such as a serving front end. */
+int cu_host_alloc(void **ptr, size_t bytes);
+int cu_host_free(void *ptr);
 
 /* cuLaunchKernel on the null stream (cudarenderer/CudaFractalRenderer.java:371-378);
  * `params` is the JCuda-style array of pointers to each argument. */

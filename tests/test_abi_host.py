@@ -164,3 +164,15 @@ def test_replay_host_built(cu):
     assert os.access(exe, os.X_OK)
     p = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=30)
     assert p.returncode == 2 and "usage" in p.stdout
+
+
+def test_host_alloc_arguments(cu):
+    """cu_host_alloc rejects a NULL out-pointer and a zero size before any device
+    call; cu_host_free(NULL) is a no-op."""
+    p = C.c_void_p()
+    with pytest.raises(cu.IllegalArgumentError):
+        cu.abi.call("cu_host_alloc", None, C.c_size_t(64))
+    with pytest.raises(cu.IllegalArgumentError):
+        cu.abi.call("cu_host_alloc", C.byref(p), C.c_size_t(0))
+    assert not p.value
+    cu.abi.call("cu_host_free", None)

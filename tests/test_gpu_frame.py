@@ -77,6 +77,26 @@ def test_pass1_iterations_are_its_own(gpu, engine, oracle):
     assert 0.5 * want_it < first[24] < first[1] < want_it
 
 
+def test_rgba_read_back_into_page_locked_memory(gpu, engine):
+    """A frame's RGBA image copied into cu_host_alloc memory equals the same image
+    copied into ordinary host memory; sizes past either buffer are refused."""
+    cu = gpu
+    seg = cu.plane_segment_from_center(*C2_CENTER, 1e-4 / 8, 256, 192)
+    tile = cu.make_tile(*seg, 256, 192, 2000, 256.0, spp=4, pattern=cu.SAMPLE_JITTER, seed=3)
+    palette = ((np.arange(256, dtype=np.uint32) * 0x00010305) | 0xFF000000).astype(np.uint32)
+    *_, rgba = render(cu, engine, [tile], 64, -1, cu.FrameTune(16, 24, 3, 0, 0), palette)
+    dev = cu.DeviceBuffer.from_array(rgba)
+    host = cu.HostBuffer(rgba.nbytes)
+    try:
+        host.download_from(dev)
+        assert np.array_equal(host.array(np.uint32), rgba)
+        with pytest.raises(ValueError):
+            host.download_from(dev, rgba.nbytes + 4)
+    finally:
+        host.close()
+        dev.close()
+
+
 def test_c1_frame_pipeline_matches_reference(gpu, engine):
     cu = gpu
     with open(os.path.join(GOLDEN, "reference_iteration.json")) as f:
