@@ -9,7 +9,8 @@ Inputs (tile descriptors, palette) are resident on the device before timing.
 value = Giga-iterations/s over the timed steps (iterations executed; an
 interior sample counts maxIter).
 
-Also reported (variants): the 256-tile uneven batch (autotuned, captured
+Also reported (variants): the reference's CPU-runnable 1024x1024 frame (maxIter
+1000) on the GPU and in full through the oracle on the host, the 256-tile uneven batch (autotuned, captured
 replay), 16 zoom sessions through the continuous-batching loop (p50/p99
 arrival-to-completion latency), the drop-in FractalRenderer path at 3840x2160
 (renderQuality and renderFast), the 1920x1080 deep-zoom frames (FP32 and FP64-delta perturbation and
@@ -348,6 +349,41 @@ def cpu_baseline(cu, tile, seconds_target=15.0):
                       f"g++ -O2 -ffp-contract=off, OpenMP {threads} threads"}
 
 
+def c1_leg(cu, engine, cpu: bool):
+    """The reference's CPU-runnable case: the 1024x1024 FP64 frame, centre (-0.5, 0),
+    width 3, 1 sample at each pixel centre, maxIter 1000, bailout 4 (counts and
+    smooth values), through the frame pipeline; and, unless cpu is False, the
+    same frame in full through the oracle on all host threads."""
+    tile = cu.make_tile(-2.0, -1.5, 1.0, 1.5, 1024, 1024, 1000, 4.0)
+    fr = cu.Frame(engine, [tile], 64, -1)
+    tn = cu.FrameTune(*TUNE, 0, 0)
+    for _ in range(3):
+        fr.render(tn)
+    ms = []
+    for _ in range(20):
+        fr.render(tn)
+        ms.append(fr.last_ms()[1])
+    it = fr.iterations()
+    counts = fr.counts()
+    fr.close()
+    m = float(np.median(ms))
+    out = {"iterations": it, "render_ms_p50": round(m, 4), "giga_iter_s": round(it / m / 1e6, 1)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        dts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            want, _, cit = _oracle.escape_tile(tile, 64, threads=threads)
+            dts.append(time.perf_counter() - t0)
+        out["cpu_oracle"] = {"ms_p50": round(float(np.median(dts)) * 1e3, 2),
+                             "giga_iter_s": round(cit / float(np.median(dts)) / 1e9, 3),
+                             "threads": threads, "same_counts": bool(np.array_equal(
+                                 counts.reshape(1024, 1024), want))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -431,6 +467,9 @@ def main():
 
     variants = {}
     if not args.no_variants:
+        log("[bench] C1 frame (1024x1024, maxIter 1000), GPU and the oracle on the host")
+        variants["c1_plain_1024"] = c1_leg(cu, engine, cpu=not args.no_cpu)
+        log(f"[bench] C1: {variants['c1_plain_1024']}")
         vsteps = max(3, args.steps // 2)
         g = GridRun(cu, engine, tile, 64)
         dt2, it2, _ = timed(cu, g, vsteps, 1)
