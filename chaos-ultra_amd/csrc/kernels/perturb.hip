@@ -14,8 +14,8 @@
 // each entry {Zx, Zy, |Z|^2/2, 0}.  Deltas are FP32 (twice the FP64 issue rate
 // on gfx950) or FP64 (far more precise).
 //
-// Rebasing (Zhuoran): when |z| < |d| — tested as fma(Zx, dx, fma(Zy, dy,
-// |Z|^2/2)) < 0 — or when the pixel outlives the reference (m == M), the pixel
+// Rebasing (Zhuoran): when |z| < |d| — tested as fma(Zx, dx, Zy dy) <
+// -|Z|^2/2 — or when the pixel outlives the reference (m == M), the pixel
 // restarts the reference from Z_0 = 0 with d = z.  Without rebasing,
 // Pauldelbrot's test |z|^2 < eps |Z|^2 marks a pixel glitched and the host
 // re-renders glitched pixels around a secondary reference.
@@ -162,6 +162,17 @@ __device__ __forceinline__ void deep_offset(const DeepArgs &a, uint px, uint py,
     oy = ((double)a.height * 0.5 - ((double)py + 0.5)) * a.pixel_h;
 }
 
+// The rebase test |Z + d| < |d|, i.e. |Z|^2 + 2 Z.d < 0, evaluated as
+// Zx dx + Zy dy < -h with h = |Z|^2 / 2 from the orbit entry: one product, one
+// FMA and a compare.  Every operand taken from the orbit enters a different
+// instruction, so a wave that reads the orbit through scalar registers needs no
+// copies into vector registers.  At Z = 0 (m = 0, h = 0) the sum is a zero and
+// the test does not fire for finite d.
+template <class Real>
+__device__ __forceinline__ bool rebase_test(Real Zx, Real Zy, Real h, Real dx, Real dy) {
+    return cu::fma_r(Zx, dx, Zy * dy) < -h;
+}
+
 // One checked step of the perturbation loop (the oracle's or_perturb_pixel
 // body).  Returns 0 to continue, 1 escaped (mag set), 2 max_iter, 3 glitched.
 template <class Real>
@@ -179,8 +190,7 @@ __device__ __forceinline__ int checked_step(const OrbitView<Real> &Z, uint M, Re
         return 1;
     }
     if (rebase) {
-        const Real t = cu::fma_r(Zx, dx, cu::fma_r(Zy, dy, e.h));
-        if (t < (Real)0 || m == M) {
+        if (rebase_test(Zx, Zy, e.h, dx, dy) || m == M) {
             dx = zx;
             dy = zy;
             m = 0;
@@ -218,8 +228,7 @@ __device__ __forceinline__ uint64_t unchecked_block(Ptr zp, Real &dx, Real &dy, 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const OrbitEntry<Real> e = load_entry<Real>(zp + k);
-        const Real t = cu::fma_r(e.x, dx, cu::fma_r(e.y, dy, e.h));
-        flagged |= cu::ballot(t < (Real)0);
+        flagged |= cu::ballot(rebase_test(e.x, e.y, e.h, dx, dy));
         const Real ax = cu::fma_r((Real)2, e.x, dx);
         const Real ay = cu::fma_r((Real)2, e.y, dy);
         const Real ndx = cu::fma_r(ax, dx, cu::fma_r(-ay, dy, dcx));
@@ -242,8 +251,7 @@ __device__ __forceinline__ uint stopping_block(Ptr zp, Real &dx, Real &dy, Real 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const OrbitEntry<Real> e = load_entry<Real>(zp + k);
-        const Real t = cu::fma_r(e.x, dx, cu::fma_r(e.y, dy, e.h));
-        fired = cu::ballot(t < (Real)0) & running;
+        fired = cu::ballot(rebase_test(e.x, e.y, e.h, dx, dy)) & running;
         if (fired != 0) {
             end = e;
             return (uint)k;

@@ -100,7 +100,7 @@ inline double fma_r(double a, double b, double c) { return std::fma(a, b, c); }
 // One pixel of the perturbation (the checked form; the gfx950 kernel's
 // unchecked blocks replay to exactly this sequence).  flags bit 0 = rebase
 // (Zhuoran: restart the reference when |z| < |d|, tested as
-// fma(Zx, dx, fma(Zy, dy, |Z|^2/2)) < 0); without it, Pauldelbrot's test
+// fma(Zx, dx, Zy * dy) < -|Z|^2/2); without it, Pauldelbrot's test
 // |z|^2 < glitch_eps * |Z|^2 (m > 0) stops the pixel as glitched.  Either way
 // the reference restarts from Z_0 when the pixel outlives it (m == M).
 // Returns the iterations executed; *state = 1 escaped, 2 max_iter, 3 glitched.
@@ -128,8 +128,7 @@ uint32_t perturb_pixel(const Real *orbit, uint32_t M, Real dcx, Real dcy, uint32
             return n;
         }
         if (rebase) {
-            const Real t = fma_r(Zx, dx, fma_r(Zy, dy, h));
-            if (t < (Real)0 || m == M) {
+            if (fma_r(Zx, dx, Zy * dy) < -h || m == M) {
                 dx = zx;
                 dy = zy;
                 m = 0;

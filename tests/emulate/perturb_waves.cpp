@@ -52,7 +52,7 @@ struct Model {
     Real zh(uint32_t m) const { return Z[4 * m + 2]; }
 
     bool fires(const Lane<Real> &L, uint32_t m) const {
-        return fma_r(zx(m), L.dx, fma_r(zy(m), L.dy, zh(m))) < (Real)0;
+        return fma_r(zx(m), L.dx, zy(m) * L.dy) < -zh(m);
     }
     void step(Lane<Real> &L, uint32_t m) const {
         const Real ax = fma_r((Real)2, zx(m), L.dx), ay = fma_r((Real)2, zy(m), L.dy);
@@ -72,7 +72,7 @@ struct Model {
             L.mag = mag;
             return 1;
         }
-        if (fma_r(Zx, L.dx, fma_r(Zy, L.dy, h)) < (Real)0 || L.m == M) {
+        if (fma_r(Zx, L.dx, Zy * L.dy) < -h || L.m == M) {
             L.dx = x;
             L.dy = y;
             L.m = 0;
