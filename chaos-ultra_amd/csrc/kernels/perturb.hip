@@ -241,9 +241,10 @@ __device__ __forceinline__ uint64_t unchecked_block(Ptr zp, Real &dx, Real &dy, 
 
 // The same steps, stopping at the first step k0 at which some lane's rebase test
 // fires: every lane is left in its state at step k0, `fired` holds the lanes whose
-// test fired there and `end` is orbit entry zp[k0].  Returns k0.  Only lanes in
-// `running` count.  Only run after unchecked_block reported a firing of a
-// running lane in the block, so k0 < K.
+// test fired there and `end` is orbit entry zp[k0].  Returns k0 (K, with `end`
+// = zp[K], when no test fires).  Only lanes in `running` count.  Its branches keep
+// the orbit reads from being issued ahead, so it runs after unchecked_block
+// reported a firing, or directly while rebases come in consecutive blocks.
 template <class Real, int K, class Ptr>
 __device__ __forceinline__ uint stopping_block(Ptr zp, Real &dx, Real &dy, Real dcx, Real dcy,
                                                uint64_t running, uint64_t &fired,
@@ -279,7 +280,8 @@ __device__ __forceinline__ uint stopping_block(Ptr zp, Real &dx, Real &dy, Real 
 // running lane cannot take a block, every running lane takes one checked step.
 //
 // A block ends at step k0 for the whole wave: K, or the first step at which a
-// running lane's rebase test fired.  There the escape test runs on z = Z_m + d.
+// running lane's rebase test fired (found by rerunning the block in the form that
+// stops there, or by running that form directly after two blocks that stopped).  There the escape test runs on z = Z_m + d.
 // Escaped (at step k0, hence at some step of the block since escape is
 // permanent): the lane replays the block with checked steps from its start.
 // Otherwise the lanes whose test fired take the rebase d = z, m = 0 with n
@@ -296,6 +298,7 @@ __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dc
     uint m = 0, n = 0;
     int state = active ? 0 : 4;
     mag_out = 0;
+    uint streak = 0;  // wave-uniform: consecutive blocks that stopped at a rebase
     for (;;) {
         const uint64_t running = cu::ballot(state == 0);
         if (running == 0) break;
@@ -307,23 +310,31 @@ __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dc
             const uint mu = __builtin_amdgcn_readlane(m, (int)__builtin_ctzll(running));
             const bool same = (cu::ballot(m != mu) & running) == 0;
             const bool lds = (cu::ballot(m + (uint)K >= Z.staged) & running) == 0;
-            uint64_t flagged;
             OrbitEntry<Real> e;
-            if (same) {
-                e = load_entry<Real>(Z.global + (mu + (uint)K));
-                flagged = unchecked_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy);
-            } else if (lds) {
-                e = load_entry<Real>(Z.lds + (m + (uint)K));
-                flagged = unchecked_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy);
-            } else {
-                e = load_entry<Real>(Z.global + (m + (uint)K));
-                flagged = unchecked_block<Real, K>(Z.global + m, dx, dy, dcx, dcy);
-            }
-            uint k0 = K;
             uint64_t fired = 0;
-            if ((flagged & running) != 0) {  // a rebase test fired: rerun the block up to there
-                dx = sx;
-                dy = sy;
+            uint k0 = K;
+            // after two blocks in a row that stopped at a rebase, rebases come often
+            // here: go straight to the form that stops at the first one
+            bool stop = streak >= 2;
+            if (!stop) {
+                uint64_t flagged;
+                if (same) {
+                    e = load_entry<Real>(Z.global + (mu + (uint)K));
+                    flagged = unchecked_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy);
+                } else if (lds) {
+                    e = load_entry<Real>(Z.lds + (m + (uint)K));
+                    flagged = unchecked_block<Real, K>(Z.lds + m, dx, dy, dcx, dcy);
+                } else {
+                    e = load_entry<Real>(Z.global + (m + (uint)K));
+                    flagged = unchecked_block<Real, K>(Z.global + m, dx, dy, dcx, dcy);
+                }
+                if ((flagged & running) != 0) {  // a rebase test fired: rerun up to there
+                    dx = sx;
+                    dy = sy;
+                    stop = true;
+                }
+            }
+            if (stop) {
                 if (same)
                     k0 = stopping_block<Real, K>(Z.global + mu, dx, dy, dcx, dcy, running, fired, e);
                 else if (lds)
@@ -332,6 +343,7 @@ __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dc
                     k0 = stopping_block<Real, K>(Z.global + m, dx, dy, dcx, dcy, running, fired,
                                                  e);
             }
+            streak = k0 < (uint)K ? streak + 1 : 0;
             if (state == 0) {
                 const Real zx = e.x + dx;
                 const Real zy = e.y + dy;
