@@ -281,15 +281,16 @@ __device__ __forceinline__ uint stopping_block(Ptr zp, Real &dx, Real &dy, Real 
 //
 // A block ends at step k0 for the whole wave: K, or the first step at which a
 // running lane's rebase test fired (found by rerunning the block in the form that
-// stops there, or by running that form directly after two blocks that stopped).  There the escape test runs on z = Z_m + d.
-// Escaped (at step k0, hence at some step of the block since escape is
-// permanent): the lane replays the block with checked steps from its start.
-// Otherwise the lanes whose test fired take the rebase d = z, m = 0 with n
-// counting the k0 steps taken, and the others simply stand at step k0.  The
-// checked loop would take the same rebase at the same state and then step from
-// Z_0 = 0; the next block starts there, and its first rebase test, at m = 0 with
-// Z_0 = |Z_0|^2 = 0, gives t = 0 and does not fire, while lanes that did not
-// fire repeat a test that passed — so every block advances.
+// stops there, or by running that form directly after two blocks that stopped).
+// There the escape test runs on z = Z_m + d.  Escaped (at step k0, hence at some
+// step of the block since escape is permanent): the lane replays the block with
+// checked steps from its start.  Otherwise the lanes whose test fired take the
+// rebase d = z, m = 0 with n counting the k0 steps taken, and the others simply
+// stand at step k0.  The checked loop would take the same rebase at the same
+// state and then step from Z_0 = 0; the next block starts there, and its first
+// rebase test, at m = 0 with Z_0 = 0 and |Z_0|^2 = 0, compares a zero with -0
+// and does not fire, while lanes that did not fire repeat a test that passed —
+// so every block advances.
 template <class Real, int K>
 __device__ int perturb_pixel(const OrbitView<Real> &Z, uint M, Real dcx, Real dcy, uint max_iter,
                              Real bsq, bool rebase, Real geps, bool active, uint &n_out,
