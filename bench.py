@@ -342,8 +342,17 @@ def cpu_baseline(cu, tile, seconds_target=15.0):
         _, _, it = _oracle.escape_tile(tile, 64, rows=(a, a + rows), threads=threads)
         dts.append(time.perf_counter() - t0)
         rates.append(it / dts[-1] / 1e9)
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": round(float(np.median(rates)), 4), "unit": "Giga-iter/s", "cores": threads,
-            "kind": "port",
+            "cpu_model": cpu_model, "kind": "port",
             "sample": f"rows {a}..{a + rows - 1} of the 4096x4096 FP64 frame (4 spp, maxIter 5000), "
                       f"{it} iterations per run, median of 3 runs ({', '.join(f'{d:.2f}' for d in dts)} s); "
                       f"g++ -O2 -ffp-contract=off, OpenMP {threads} threads"}
