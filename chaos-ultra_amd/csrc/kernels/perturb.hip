@@ -48,19 +48,32 @@ __device__ __forceinline__ DD dd_quick_two_sum(double a, double b) {
     const double s = a + b;
     return {s, b - (s - a)};
 }
-__device__ __forceinline__ DD dd_add(DD a, DD b) {
-    DD s = dd_two_sum(a.hi, b.hi);
-    const DD t = dd_two_sum(a.lo, b.lo);
-    double e = s.lo + t.hi;
-    s = dd_quick_two_sum(s.hi, e);
-    e = s.lo + t.lo;
-    return dd_quick_two_sum(s.hi, e);
+
+__device__ __forceinline__ DD dd_two_prod(double a, double b) {
+    const double p = a * b;
+    return {p, __builtin_fma(a, b, -p)};  // double operands
 }
-__device__ __forceinline__ DD dd_mul(DD a, DD b) {
-    const double p = a.hi * b.hi;
-    const double pe = __builtin_fma(a.hi, b.hi, -p);  // double operands
-    const double cross = a.hi * b.lo + a.lo * b.hi;
-    return dd_quick_two_sum(p, pe + cross);
+
+// One step Z <- Z^2 + C of the reference orbit, compensated rather than composed
+// from dd_mul and dd_add: the leading products are split exactly, the sums of
+// leading terms are exact (two-sums), and the low-order terms (the products'
+// errors, the cross terms 2 xh xl - 2 yh yl and 2 (xh yl + xl yh), C's low parts)
+// are summed in plain double before one renormalisation; terms of order xl^2 are
+// dropped.  The absolute error per step is a few units of 2^-104 |Z|^2, as for
+// the composed form, in about half its instructions: the orbit is one lane's
+// serial chain, issued one instruction at a time.
+__device__ __forceinline__ void dd_orbit_step(DD &x, DD &y, DD cx, DD cy) {
+    const DD px = dd_two_prod(x.hi, x.hi), py = dd_two_prod(y.hi, y.hi);
+    const DD pxy = dd_two_prod(x.hi, y.hi);
+    const DD s1 = dd_two_sum(px.hi, -py.hi);
+    const DD s2 = dd_two_sum(s1.hi, cx.hi);
+    const double u = __builtin_fma(x.hi, x.lo, -(y.hi * y.lo));
+    const double xlo = ((s1.lo + s2.lo) + (px.lo - py.lo)) + ((u + u) + cx.lo);
+    const DD s3 = dd_two_sum(pxy.hi + pxy.hi, cy.hi);
+    const double w = __builtin_fma(x.hi, y.lo, x.lo * y.hi);
+    const double ylo = (s3.lo + (pxy.lo + pxy.lo)) + ((w + w) + cy.lo);
+    x = dd_quick_two_sum(s2.hi, xlo);
+    y = dd_quick_two_sum(s3.hi, ylo);
 }
 
 template <class Real>
@@ -121,12 +134,7 @@ __device__ void reference_orbit(double cx_hi, double cx_lo, double cy_hi, double
         e.pad = 0;
         orbit[n] = e;
         if (!(mag < bailout_sq) || n == max_iter) break;
-        const DD xx = dd_mul(x, x);
-        const DD yy = dd_mul(y, y);
-        const DD xy = dd_mul(x, y);
-        const DD xn = dd_add(dd_add(xx, DD{-yy.hi, -yy.lo}), cx);
-        y = dd_add(DD{xy.hi + xy.hi, xy.lo + xy.lo}, cy);
-        x = xn;
+        dd_orbit_step(x, y, cx, cy);
         ++n;
     }
     *length = n;

@@ -65,6 +65,29 @@ inline DD dd_mul(DD a, DD b) {
 }
 inline DD dd_twice(DD a) { return {a.hi + a.hi, a.lo + a.lo}; }
 
+// One step Z <- Z^2 + C of the reference orbit in double-double, compensated
+// rather than composed from dd_mul and dd_add: the leading products are split
+// exactly (two_prod), the sums of leading terms are exact (two_sum), and the
+// low-order terms (the products' errors, the cross terms 2 xh xl - 2 yh yl and
+// 2 (xh yl + xl yh), C's low parts) are summed in plain double before one
+// renormalisation.  Terms of order xl^2 are dropped.  The absolute error per step
+// is a few units of 2^-104 |Z|^2, the accuracy of the composed form, in about
+// half its operations (47 against 95).
+inline void orbit_step(DD &x, DD &y, DD cx, DD cy) {
+    const DD px = two_prod(x.hi, x.hi), py = two_prod(y.hi, y.hi), pxy = two_prod(x.hi, y.hi);
+    // x' = xh^2 - yh^2 + cxh + [ex - ey + 2 (xh xl - yh yl) + cxl]
+    const DD s1 = two_sum(px.hi, -py.hi);
+    const DD s2 = two_sum(s1.hi, cx.hi);
+    const double u = std::fma(x.hi, x.lo, -(y.hi * y.lo));
+    const double xlo = ((s1.lo + s2.lo) + (px.lo - py.lo)) + ((u + u) + cx.lo);
+    // y' = 2 xh yh + cyh + [2 exy + 2 (xh yl + xl yh) + cyl]
+    const DD s3 = two_sum(pxy.hi + pxy.hi, cy.hi);
+    const double w = std::fma(x.hi, y.lo, x.lo * y.hi);
+    const double ylo = (s3.lo + (pxy.lo + pxy.lo)) + ((w + w) + cy.lo);
+    x = quick_two_sum(s2.hi, xlo);
+    y = quick_two_sum(s3.hi, ylo);
+}
+
 // Reference orbit Z_0 = 0, Z_{n+1} = Z_n^2 + C in double-double, stored rounded
 // to Real: entry n = {Real(X.hi), Real(Y.hi), Real(0.5*(X.hi^2 + Y.hi^2)), 0}.
 // Stops after storing the first Z_n with X.hi^2 + Y.hi^2 >= bailout_sq, or Z_max_iter.
@@ -84,12 +107,7 @@ uint32_t reference_orbit(double cx_hi, double cx_lo, double cy_hi, double cy_lo,
         orbit[4 * n + 2] = (Real)(0.5 * mag);
         orbit[4 * n + 3] = (Real)0;
         if (!(mag < bailout_sq) || n == max_iter) return n;
-        const DD xx = dd_mul(x, x);
-        const DD yy = dd_mul(y, y);
-        const DD xy = dd_mul(x, y);
-        const DD xn = dd_add(dd_add(xx, dd_neg(yy)), cx);
-        y = dd_add(dd_twice(xy), cy);
-        x = xn;
+        orbit_step(x, y, cx, cy);
         ++n;
     }
 }

@@ -62,7 +62,12 @@ def test_reference_orbit_follows_300_bit_orbit():
         true_m = None
         for n in range(50001):
             err = max(abs(float(x) - orbit[n, 0]), abs(float(y) - orbit[n, 1])) if n <= M else 0
-            assert err <= 2.3e-16 * max(1.0, abs(float(x)), abs(float(y))), (n, err)
+            # within 2 ulp before the escape; the escaped entry (|Z| ~ 56, past the
+            # bailout, used only by the escape test and the smooth value) carries the
+            # orbit's accumulated double-double error amplified by the last steps:
+            # within 4 ulp there
+            tol = 2.3e-16 if n < M else 4.6e-16
+            assert err <= tol * max(1.0, abs(float(x)), abs(float(y))), (n, err)
             if x * x + y * y >= 256:
                 true_m = n
                 break
