@@ -72,6 +72,25 @@ def test_perturbation_matches_oracle(gpu, engine, width, method):
     assert_smooth_close(smooth.ravel(), smooth_of(wc, wm, ws))
 
 
+# frame sizes that leave partial 16x16 / 32x16 blocks and partial 8x8 waves in
+# both directions, and budgets shorter than one 16-step block (checked steps only)
+# or just past it
+@pytest.mark.parametrize("w,h,max_iter", [(37, 23, 50000), (1, 1, 50000), (130, 9, 15),
+                                          (65, 70, 17)])
+@pytest.mark.parametrize("method", [32, 64])
+def test_perturbation_ragged_frames_and_short_budgets(gpu, engine, w, h, max_iter, method):
+    cu = gpu
+    d = deep(cu, 1e-6, w=w, h=h, max_iter=max_iter)
+    counts, smooth, glitch, rep, orbit = render(cu, engine, d, method)
+    assert counts.shape == (h, w)
+    ys, xs = np.mgrid[0:h, 0:w]
+    om = O.DEEP_PERTURB_F32 if method == 32 else O.DEEP_PERTURB_F64
+    wc, wm, ws, wit = O.deep_pixels(om, xs.ravel(), ys.ravel(), d, orbit, rep.reference_length)
+    assert np.array_equal(counts.ravel(), wc)
+    assert rep.iterations == wit
+    assert_smooth_close(smooth.ravel(), smooth_of(wc, wm, ws))
+
+
 def test_direct_matches_oracle(gpu, engine):
     cu = gpu
     d = deep(cu, 1e-10)
