@@ -6,7 +6,9 @@ Candidates:
     pixel block a wave covers: 8x8 .. 64x1 pixels at 1 spp), unchecked block
     length, hand-off threshold (lanes still running when a wave hands its
     stragglers to the queue), number of tail passes, and whether one persistent
-    drain launch replaces the tail passes (its idle-lane refill threshold);
+    drain launch runs what the tail passes leave instead of the last of them
+    running it to the end (its idle-lane refill threshold), or the whole queue
+    with no tail pass;
   * the persistent lane-refill kernel (cu_escape_tiles_persistent): block
     length, resident waves per CU, pixels taken per queue grab and the number
     of idle lanes that trigger a refill.
@@ -31,6 +33,7 @@ PIPELINE_SPACE = {
 }
 PIPELINE_START = {"block_steps": 16, "handoff_lanes": 16, "tail_passes": 5, "wave_shape": 0,
                   "drain_refill": 0}
+DRAIN_ONLY_REFILL = (16, 32)  # tried after the descent with tail_passes 0
 
 PERSISTENT_SPACE = {
     "block_steps": (16, 8, 32),
@@ -70,6 +73,22 @@ def _descend(space, start, measure, log):
                 log(f"[autotune] {cand} {ms:.3f} ms")
             if ms < best_ms:
                 best, best_ms = cand, ms
+    return best, best_ms, table
+
+
+def search_pipeline(measure, log=None):
+    """Coordinate descent over PIPELINE_SPACE, then the drain over the whole
+    pass-1 queue (tail_passes 0), which one knob at a time cannot reach: tail
+    passes 0 needs a drain.  `measure(knobs) -> ms`; returns (best, ms, table)."""
+    best, best_ms, table = _descend(PIPELINE_SPACE, PIPELINE_START, measure, log)
+    for refill in DRAIN_ONLY_REFILL:
+        cand = dict(best, tail_passes=0, drain_refill=refill)
+        ms = measure(cand)
+        table.append((cand, ms))
+        if log:
+            log(f"[autotune] {cand} {ms:.3f} ms")
+        if ms < best_ms:
+            best, best_ms = cand, ms
     return best, best_ms, table
 
 
@@ -114,7 +133,7 @@ def tune_batch(cu, engine, tiles, precision=64, repeats=3, warmup=1, persistent=
             return frame.last_ms()[1]
         return _median_ms(one, repeats, warmup)
 
-    pbest, pms, ptable = _descend(PIPELINE_SPACE, PIPELINE_START, measure_pipeline, log)
+    pbest, pms, ptable = search_pipeline(measure_pipeline, log)
     result = {"pipeline": {"best": pbest, "ms": pms, "table": ptable}}
     frame.close()
     best = ("pipeline", pbest, pms)

@@ -115,6 +115,8 @@ cu_frame_tune checked_tune(const cu_frame_tune *tune, const uint32_t *d_palette,
     if (tn.drain_refill > 64) fail(CU_ERR_ILLEGAL_ARGUMENT, "drain_refill must be in [0, 64]");
     if (tn.drain_refill == 0 && (tn.tail_passes == 0 || tn.tail_passes > kMaxTailPasses))
         fail(CU_ERR_ILLEGAL_ARGUMENT, "tail_passes must be in [1, 8]");
+    if (tn.drain_refill != 0 && tn.tail_passes >= kMaxTailPasses)
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "tail_passes must be in [0, 7] before a drain");
     if (tn.wave_shape >= kWaveShapes) fail(CU_ERR_ILLEGAL_ARGUMENT, "wave_shape must be in [0, 3]");
     if ((d_rgba == nullptr) != (d_palette == nullptr))
         fail(CU_ERR_ILLEGAL_ARGUMENT, "palette and rgba go together");
@@ -168,21 +170,14 @@ void enqueue_render(cu_frame f, const cu_frame_tune &tn, const uint32_t *d_palet
     }
     if (events) check(hipEventRecord(f->ev_pass1, st), "hipEventRecord");
     const uint32_t cus = (uint32_t)std::max(1, compute_units());
-    if (tn.drain_refill) {
-        PipeArgs ap = a;
-        ap.iters = (unsigned long long *)f->d_iters + CU_ITER_SLOTS;
-        const void *q = f->d_queue[0];
-        const uint32_t *fills = counters;
-        uint32_t *grab = counters + (kMaxTailPasses + 1) * kSubQueues;
-        uint32_t refill = tn.drain_refill;
-        void *params[] = {&ap, &q, &fills, &grab, &refill};
-        // 256 threads per block: the kernel's queue prefix assumes it
-        launch(f->engine, drain, dim3(cus * 5), dim3(256), params, stream, false);
-    }
-    for (uint32_t p = 1; !tn.drain_refill && p <= tn.tail_passes; ++p) {
+    // tail passes p = 1..tail_passes: pass p reads queue (p-1) & 1 with fill set
+    // p-1 and writes queue p & 1 with fill set p.  Without a drain the last of
+    // them runs every orbit to its end; with one, all hand off and the drain
+    // takes what the last left.
+    for (uint32_t p = 1; p <= tn.tail_passes; ++p) {
         const uint32_t in = (p - 1) & 1u, outq = p & 1u;
         PipeArgs ap = a;
-        ap.handoff = p == tn.tail_passes ? 0u : tn.handoff_lanes;
+        ap.handoff = (!tn.drain_refill && p == tn.tail_passes) ? 0u : tn.handoff_lanes;
         ap.iters = (unsigned long long *)f->d_iters + CU_ITER_SLOTS;
         const void *qin = f->d_queue[in];
         const uint32_t *cin = counters + (p - 1) * kSubQueues;
@@ -190,6 +185,17 @@ void enqueue_render(cu_frame f, const cu_frame_tune &tn, const uint32_t *d_palet
         uint32_t *cout = counters + p * kSubQueues;
         void *params[] = {&ap, &qin, &cin, &qout, &cout};
         launch(f->engine, passq, dim3(cus * 8), dim3(256), params, stream, false);
+    }
+    if (tn.drain_refill) {
+        PipeArgs ap = a;
+        ap.iters = (unsigned long long *)f->d_iters + CU_ITER_SLOTS;
+        const void *q = f->d_queue[tn.tail_passes & 1u];
+        const uint32_t *fills = counters + tn.tail_passes * kSubQueues;
+        uint32_t *grab = counters + (kMaxTailPasses + 1) * kSubQueues;
+        uint32_t refill = tn.drain_refill;
+        void *params[] = {&ap, &q, &fills, &grab, &refill};
+        // 256 threads per block: the kernel's queue prefix assumes it
+        launch(f->engine, drain, dim3(cus * 5), dim3(256), params, stream, false);
     }
     {
         uint64_t total = f->total_pixels;

@@ -342,12 +342,13 @@ int cu_engine_last_kernel_ms(cu_engine e, float *ms);
 typedef struct cu_frame_tune {
     uint32_t block_steps;   /* 8 or 16 */
     uint32_t handoff_lanes; /* 1..32 */
-    uint32_t tail_passes;   /* 1..8 */
+    uint32_t tail_passes;   /* 1..8; with a drain 0..7, the hand-off passes run before it */
     uint32_t wave_shape;    /* 0..3: pass-1 wave covers a bw x bh pixel block, bw * bh = 64/spp;
                                0 near-square (spp 1: 8x8), each step twice as wide (16x4, 32x2, 64x1) */
-    uint32_t drain_refill;  /* 0: the queue runs through tail_passes launches; 1..64: one persistent
-                               drain launch runs it to the end, refilling a wave's idle lanes once
-                               this many wait (tail_passes is then ignored) */
+    uint32_t drain_refill;  /* 0: the queue runs through tail_passes launches, the last to the end;
+                               1..64: every tail pass hands off and one persistent drain launch
+                               runs what is left to the end, refilling a wave's idle lanes once
+                               this many wait */
 } cu_frame_tune;
 typedef struct cu_frame_s *cu_frame;
 int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, int precision,

@@ -55,6 +55,33 @@ def test_orbit_bit_exact(gpu, engine, method, precision):
     assert orbit.shape == want.shape and np.array_equal(orbit, want)
 
 
+# Centres with low parts, on the real axis (y stays zero: signs of zero), inside
+# the set (the orbit runs to max_iter) and escaping after a few steps; every
+# entry compared bit for bit with the oracle's.
+@pytest.mark.parametrize("centre,lo", [
+    ((-0.743643887037151, 0.131825904205330), (1.3e-17, -2.1e-17)),
+    ((-1.0, 0.0), (0.0, 0.0)),
+    ((-1.25, 0.0), (0.0, 0.0)),
+    ((0.3, -0.5), (0.0, 0.0)),
+    ((-0.12, 0.74), (0.0, 0.0)),
+    ((0.26, 1e-3), (-4e-20, 3e-20)),
+])
+@pytest.mark.parametrize("method,precision", [(32, 32), (64, 64)])
+def test_orbit_bits_across_centres(gpu, engine, centre, lo, method, precision):
+    cu = gpu
+    d = cu.make_deep(centre[0], centre[1], 1e-12, 8, 8, 20000, centre_lo=lo)
+    f = cu.DeepFrame(engine, d, method)
+    try:
+        f.render()
+        got, rep = f.orbit(), f.report()
+    finally:
+        f.close()
+    want, M = O.reference_orbit(centre[0], centre[1], 20000, 256.0, precision, lo)
+    assert rep.reference_length == M
+    assert got.shape == want.shape
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
 # 1.0: a shallow frame around one reference, where rebases come every few steps
 # and most blocks stop early at a rebase
 @pytest.mark.parametrize("width", [1e-6, 1e-10, 1e-20, 1.0])

@@ -38,9 +38,10 @@ def render(cu, engine, tiles, precision, variant, tune, palette=None):
 
 
 # (block_steps, handoff_lanes, tail_passes, drain_refill): drain_refill > 0 runs
-# the queue in one persistent launch instead of tail passes
+# what the tail passes (all handing off) leave in one persistent launch
 @pytest.mark.parametrize("tune", [(16, 16, 3, 0), (8, 16, 3, 0), (16, 1, 1, 0), (16, 32, 8, 0),
-                                  (8, 4, 2, 0), (16, 24, 0, 16), (8, 32, 0, 64), (16, 8, 0, 1)])
+                                  (8, 4, 2, 0), (16, 24, 0, 16), (8, 32, 0, 64), (16, 8, 0, 1),
+                                  (16, 24, 2, 16), (8, 8, 7, 32)])
 @pytest.mark.parametrize("precision", [64, 32])
 def test_c2_frame_pipeline_vs_oracle(gpu, engine, oracle, tune, precision):
     cu = gpu
@@ -146,6 +147,8 @@ def test_frame_rejects(gpu, engine):
             f.render(cu.FrameTune(16, 16, 0, 0, 0))
         with pytest.raises(cu.IllegalArgumentError, match="drain_refill"):
             f.render(cu.FrameTune(16, 16, 0, 0, 65))
+        with pytest.raises(cu.IllegalArgumentError, match="before a drain"):
+            f.render(cu.FrameTune(16, 16, 8, 0, 16))
         with pytest.raises(cu.IllegalStateError):
             f.iterations()
     finally:
