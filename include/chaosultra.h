@@ -333,7 +333,8 @@ int cu_engine_last_kernel_ms(cu_engine e, float *ms);
  * workspace.  Rendering runs pass 1 over every sample (one lane each), hands the
  * orbits still running in a wave off to a queue once no more than
  * handoff_lanes of its 64 lanes iterate, drains the queue in tail_passes
- * further passes (the last one to completion), and writes the pixel planes
+ * further passes (the last one to completion, unless a persistent drain launch
+ * takes what they leave), and writes the pixel planes
  * (count: integer mean; smooth: mean summed in sample order) and, when a
  * palette is given, RGBA8.  Pixels whose samples all finish in pass 1 are
  * written there; the others are combined after the last pass.  Tiles are laid
