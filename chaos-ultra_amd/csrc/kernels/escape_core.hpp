@@ -259,6 +259,20 @@ __device__ __forceinline__ double smooth_constant(double bailout_sq) {
     return log2(log2(bailout_sq));
 }
 
+// Palette slot of a colour value: L - (round(v) mod L) - 1, round half away
+// from zero (colorize, mandelbrot.cu:27-35).  A value that is not above zero
+// takes the slot of 0: the reference's counts never are, but a smooth value is
+// negative for an orbit that starts far outside (|z0| or |c| far beyond R), and
+// converting a negative value to an unsigned index is undefined.
+__device__ __forceinline__ uint palette_slot(double v, uint length) {
+    const uint vi = v > 0.0 ? (uint)round(v) : 0u;
+    return length - (vi % length) - 1;
+}
+__device__ __forceinline__ uint palette_slot(float v, uint length) {
+    const uint vi = v > 0.0f ? (uint)roundf(v) : 0u;
+    return length - (vi % length) - 1;
+}
+
 // SplitMix64 (Steele, Lea, Flood 2014): output j of the generator seeded with
 // `seed` is mix(seed + (j+1) * 0x9E3779B97F4A7C15).
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {

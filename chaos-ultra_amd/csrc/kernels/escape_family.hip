@@ -344,7 +344,7 @@ CU_TILE_KERNELS(FloatExact, float, false)
 CU_TILE_KERNELS(FloatFused, float, true)
 
 // Palette compose of a smooth plane: idx = L - (round(v) mod L) - 1 (the
-// colorize of mandelbrot.cu:27-35 applied to the smooth value).
+// colorize of mandelbrot.cu:27-35 applied to the smooth value; v <= 0 as 0).
 extern "C" __global__ __launch_bounds__(256) void composeSmooth(
     const double *smooth, uint smooth_pitch, uint *rgba, uint rgba_pitch, uint width,
     uint height, const uint *palette, uint palette_length) {
@@ -359,8 +359,8 @@ extern "C" __global__ __launch_bounds__(256) void composeSmooth(
     for (uint y = blockIdx.y * blockDim.y + threadIdx.y; y < height; y += gridDim.y * blockDim.y)
         for (uint x = blockIdx.x * blockDim.x + threadIdx.x; x < width;
              x += gridDim.x * blockDim.x) {
-            const uint vi = (uint)round(smooth[(size_t)y * smooth_pitch + x]);
-            rgba[(size_t)y * rgba_pitch + x] = pal[palette_length - (vi % palette_length) - 1];
+            rgba[(size_t)y * rgba_pitch + x] =
+                pal[cu::palette_slot(smooth[(size_t)y * smooth_pitch + x], palette_length)];
         }
 }
 

@@ -169,8 +169,7 @@ __device__ __forceinline__ void write_pixel(const PipeArgs &a, uint lspp, uint64
     a.counts[o] = csum >> lspp;
     if (a.smooth) a.smooth[o] = sm;
     if (a.rgba) {
-        const uint vi = (uint)round(sm);
-        a.rgba[o] = a.palette[a.palette_length - (vi % a.palette_length) - 1];
+        a.rgba[o] = a.palette[cu::palette_slot(sm, a.palette_length)];
     }
 }
 

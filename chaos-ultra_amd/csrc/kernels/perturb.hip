@@ -392,8 +392,7 @@ __device__ __forceinline__ void deep_write(const DeepArgs &a, uint64_t o, int st
     if (a.smooth) a.smooth[o] = sm;
     if (a.glitch) a.glitch[o] = state == 3 ? 1 : 0;
     if (a.rgba) {
-        const uint vi = (uint)round(sm);
-        a.rgba[o] = a.palette[a.palette_length - (vi % a.palette_length) - 1];
+        a.rgba[o] = a.palette[cu::palette_slot(sm, a.palette_length)];
     }
 }
 

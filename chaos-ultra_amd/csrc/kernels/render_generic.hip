@@ -428,8 +428,7 @@ extern "C" __global__ __launch_bounds__(1024) void compose(
                 if (p.isReused) color = colorize_sample_count((uint)p.weightOfNewSamples, full);
             } else {
                 // colorize (mandelbrot.cu:27-35)
-                const uint vi = (uint)roundf(p.value);
-                color = pal[paletteLength - (vi % paletteLength) - 1];
+                color = pal[cu::palette_slot(p.value, paletteLength)];
             }
             output[(size_t)y * width + x] = color;
         }

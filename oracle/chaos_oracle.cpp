@@ -8,24 +8,36 @@
 // double arithmetic round to their own formats, no excess precision), so every
 // expression below rounds exactly as written, left to right.
 //
-// Pinning (what checks each function; see tests/test_oracle_*.py):
-//   or_mandelbrot_* / or_julia_*   bit-exact against the reference's own
-//                                  computeFractal compiled for the host
-//                                  (oracle/_ref) and committed golden vectors;
-//                                  special cases c = 0, 1, -2, 0.25; mirror and
-//                                  point symmetry; Julia = Mandelbrot - 1.
-//   or_escape_*                    equals or_mandelbrot_* at bailout 4; closed
-//                                  form escape counts for real c > 1/4 at other
-//                                  bailouts; brute-force mpmath orbits.
+// Pinning (what checks each function; tests/oracle_pins.py holds the checks,
+// tests/test_oracle_mutants.py shows each one failing on a copy of this file
+// with a planted mistake).  The reference's computeFractal cannot be compiled
+// here (its sources need CUDA toolkit headers this image lacks), so nothing is
+// checked against a build of it:
+//   or_mandelbrot_* / or_julia_*   FP32 and FP64: numpy float32/float64 brute
+//                                  force of the reference loop on grids and near
+//                                  the boundary, and on the whole C1 frame;
+//                                  dyadic special points (c = -2 stops at
+//                                  |z|^2 = 4); mirror and point symmetry;
+//                                  Julia = Mandelbrot - 1; seahorse and cusp
+//                                  asymptotics.
+//   or_escape_*, or_tile_point_*   the FP32 tile path against the float32 brute
+//                                  force; sample positions exact on dyadic grids
+//                                  and equal to float32 evaluation elsewhere;
+//                                  300-digit mpmath orbits at other bailouts.
 //   or_smooth                      mpmath on tiny inputs; continuity across
 //                                  count bands.
 //   or_jitter                      published SplitMix64 vectors.
 //   or_render_main / _advanced     special cases (1 spp = corner sample; reuse of
-//                                  an identical rectangle copies the value),
-//                                  brute-force vote groups on tiny images.
-//                                  The __activemask() membership and CUDA's atanf
-//                                  are not pinned by anything the reference holds.
-//   or_compose*                    closed form of the palette index; byte order.
+//                                  an identical rectangle copies the value).
+//                                  The __activemask() membership of the votes is
+//                                  not pinned by anything the reference holds.
+//   or_foveation                   float64 closed form at distances from the
+//                                  fovea to beyond 60 degrees (CUDA's atanf is
+//                                  taken as the double arctangent rounded once).
+//   or_compose*                    known palette slots (half-integers, values at
+//                                  and beyond L, negatives) and grey levels;
+//                                  byte order.
+//   or_render_undersampled         the set of pixels written and their values.
 //   or_default_palette             the reference's palettes/defaultPalette.png.
 #include "chaos_oracle.h"
 
@@ -462,11 +474,19 @@ uint32_t colorize_sample_count(uint32_t sample_count, uint32_t full) {
     return v | (v << 8) | (v << 16) | (0xffu << 24);
 }
 
-// colorize (mandelbrot.cu:27-35)
+// Palette slot L - (round(v) mod L) - 1 of colorize (mandelbrot.cu:27-35),
+// round half away from zero.  A value that is not above zero takes the slot
+// of 0 (the reference's values never are; a negative smooth value of an orbit
+// starting far outside would make the unsigned conversion undefined).
+uint32_t palette_slot(double v, uint32_t length) {
+    uint32_t vi = v > 0.0 ? (uint32_t)std::round(v) : 0u;
+    return length - (vi % length) - 1;
+}
+
+// colorize (mandelbrot.cu:27-35); the float value rounds as a float (roundf).
 uint32_t colorize(const uint32_t *palette, uint32_t length, float value) {
-    uint32_t vi = (uint32_t)std::round(value);
-    uint32_t idx = length - (vi % length) - 1;
-    return palette[idx];
+    uint32_t vi = value > 0.0f ? (uint32_t)std::round(value) : 0u;
+    return palette[length - (vi % length) - 1];
 }
 
 }  // namespace
@@ -612,8 +632,7 @@ void or_compose_smooth(const double *smooth, uint32_t smooth_pitch, uint32_t *ou
                        uint32_t L) {
     for (uint32_t y = 0; y < H; ++y)
         for (uint32_t x = 0; x < W; ++x) {
-            uint32_t vi = (uint32_t)std::round(smooth[(size_t)y * smooth_pitch + x]);
-            out[(size_t)y * out_pitch + x] = palette[L - (vi % L) - 1];
+            out[(size_t)y * out_pitch + x] = palette[palette_slot(smooth[(size_t)y * smooth_pitch + x], L)];
         }
 }
 

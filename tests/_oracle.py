@@ -12,7 +12,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
-REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref.so")
 
 _D, _F, _U32, _U64, _I, _P = C.c_double, C.c_float, C.c_uint32, C.c_uint64, C.c_int, C.c_void_p
 _SIG = {
@@ -59,34 +58,27 @@ PIXEL_INFO_DTYPE = np.dtype([("value", "<f4"), ("weight", "<f4"), ("is_reused", 
 _lib = None
 
 
+def bind(path: str) -> C.CDLL:
+    """Load an oracle build at `path` and declare the signatures of the symbols it
+    exports (a build of chaos_oracle.cpp alone has no deep-zoom symbols)."""
+    l = C.CDLL(path)
+    for n, (r, a) in _SIG.items():
+        try:
+            f = getattr(l, n)
+        except AttributeError:
+            continue
+        f.restype = r
+        f.argtypes = a
+    return l
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(ORACLE_LIB):
             raise ImportError(f"{ORACLE_LIB} missing: run `make -C oracle`")
-        l = C.CDLL(ORACLE_LIB)
-        for n, (r, a) in _SIG.items():
-            f = getattr(l, n)
-            f.restype = r
-            f.argtypes = a
-        _lib = l
+        _lib = bind(ORACLE_LIB)
     return _lib
-
-
-def ref_lib():
-    """The reference's own computeFractal compiled for the host (this container only)."""
-    if not os.path.exists(REF_LIB):
-        return None
-    l = C.CDLL(REF_LIB)
-    l.ref_mandelbrot_d.restype = _F
-    l.ref_mandelbrot_d.argtypes = [_U32, _D, _D]
-    l.ref_mandelbrot_f.restype = _F
-    l.ref_mandelbrot_f.argtypes = [_U32, _F, _F]
-    l.ref_julia_d.restype = _F
-    l.ref_julia_d.argtypes = [_U32, _D, _D, _D, _D]
-    l.ref_julia_f.restype = _F
-    l.ref_julia_f.argtypes = [_U32, _F, _F, _D, _D]
-    return l
 
 
 def _ptr(a: np.ndarray):

@@ -1,11 +1,9 @@
-"""The oracle against the reference's own outputs and against mathematics (CPU).
-
-Golden vectors come from the reference's computeFractal compiled for the host
-(tests/golden/make_golden.py); the remaining checks are invariants, special
-cases, closed forms and brute force chosen so that a dropped term, a wrong
-sign/index or a swapped operand in the oracle fails at least one of them.
+"""The oracle against mathematics (CPU): invariants, special cases, closed forms,
+asymptotics and exact-arithmetic orbits, chosen so that a dropped term, a wrong
+sign/index or a swapped operand in the oracle fails at least one of them.  The
+brute-force and known-answer pins (and the planted-mistake runs that show they
+bite) are in tests/test_oracle_pins.py and tests/test_oracle_mutants.py.
 """
-import hashlib
 import json
 import math
 import os
@@ -14,62 +12,6 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN
-
-
-def sha(a):
-    return hashlib.sha256(np.ascontiguousarray(a, dtype="<u4").tobytes()).hexdigest()
-
-
-@pytest.fixture(scope="module")
-def gold():
-    with open(os.path.join(GOLDEN, "reference_iteration.json")) as f:
-        return json.load(f)
-
-
-@pytest.mark.parametrize("precision", [64, 32])
-def test_c1_frame_matches_reference(cu, oracle, gold, precision):
-    g = gold[f"c1_mandelbrot_fp{precision}"]
-    tile = cu.make_tile(-2.0, -1.5, 1.0, 1.5, 1024, 1024, 1000, 4.0)
-    counts, smooth, it = oracle.escape_tile(tile, precision)
-    assert sha(counts) == g["sha256_counts_u32le"]
-    for x, y, c in g["samples"]:
-        assert counts[y, x] == c
-    interior = int((counts == 0).sum())
-    assert it == int(counts.astype(np.uint64).sum()) + 1000 * interior
-
-
-@pytest.mark.parametrize("precision", [64, 32])
-def test_julia_frame_matches_reference(cu, oracle, gold, precision):
-    g = gold[f"julia_fp{precision}"]
-    tile = cu.make_tile(-1.5, -1.5, 1.5, 1.5, 512, 512, 900, 4.0, kind=cu.FRACTAL_JULIA,
-                        julia_c=tuple(g["julia_c"]))
-    counts, _, _ = oracle.escape_tile(tile, precision)
-    assert sha(counts) == g["sha256_counts_u32le"]
-
-
-def test_points_match_reference(oracle, gold):
-    L = oracle.lib()
-    g = gold["points"]
-    n = g["max_iter"]
-    jcx, jcy = g["julia_c"]
-    for cx, cy, md, mf, jd, jf in g["rows"]:
-        assert L.or_mandelbrot_d(n, cx, cy) == md
-        assert L.or_mandelbrot_f(n, cx, cy) == mf
-        assert L.or_julia_d(n, 0.8 * cx, cy, jcx, jcy) == jd
-        assert L.or_julia_f(n, 0.8 * cx, cy, jcx, jcy) == jf
-
-
-def test_reference_build_agrees_on_fresh_points(oracle):
-    ref = oracle.ref_lib()
-    if ref is None:
-        pytest.skip("oracle/_ref not built (reference tree absent)")
-    L = oracle.lib()
-    rng = np.random.default_rng(99)
-    for _ in range(5000):
-        cx, cy = rng.uniform(-2.1, 0.7), rng.uniform(-1.2, 1.2)
-        assert L.or_mandelbrot_d(700, cx, cy) == ref.ref_mandelbrot_d(700, cx, cy)
-        assert L.or_mandelbrot_f(700, cx, cy) == ref.ref_mandelbrot_f(700, cx, cy)
-        assert L.or_julia_d(700, cx, cy, -0.4, 0.6) == ref.ref_julia_d(700, cx, cy, -0.4, 0.6)
 
 
 def test_special_points(oracle):
