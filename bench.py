@@ -320,9 +320,11 @@ def read_traffic():
         return None
 
 
-def cpu_baseline(cu, tile, seconds_target=15.0):
+def cpu_baseline(cu, tile, gpu_counts=None, gpu_smooth=None, seconds_target=15.0):
     """The oracle (a scalar C++ restatement of the reference iteration, OpenMP over rows)
-    on a bounded row band of the same frame."""
+    on a bounded row band of the same frame.  The band's counts and smooth values are
+    compared with the same rows of the GPU's frame (gpu_counts / gpu_smooth, the
+    full 4096x4096 planes of the timed render)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle
     threads = os.cpu_count() or 1
@@ -339,9 +341,19 @@ def cpu_baseline(cu, tile, seconds_target=15.0):
     rates, it, dts = [], 0, []
     for _ in range(3):  # median of 3 runs of the band
         t0 = time.perf_counter()
-        _, _, it = _oracle.escape_tile(tile, 64, rows=(a, a + rows), threads=threads)
+        band_c, band_s, it = _oracle.escape_tile(tile, 64, rows=(a, a + rows), threads=threads)
         dts.append(time.perf_counter() - t0)
         rates.append(it / dts[-1] / 1e9)
+    check = {}
+    if gpu_counts is not None:
+        g_c = gpu_counts.reshape(H, W)[a:a + rows]
+        g_s = gpu_smooth.reshape(H, W)[a:a + rows]
+        o_c, o_s = band_c[a:a + rows], band_s[a:a + rows]
+        rel = np.abs(g_s - o_s) / np.maximum(np.abs(o_s), 1.0)
+        check = {"same_counts": bool(np.array_equal(g_c, o_c)),
+                 "pixels_compared": int(g_c.size),
+                 "smooth_max_rel_diff": float(rel.max()),
+                 "smooth_within_1e-12": bool(rel.max() <= 1e-12)}
     cpu_model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -355,7 +367,8 @@ def cpu_baseline(cu, tile, seconds_target=15.0):
             "cpu_model": cpu_model, "kind": "port",
             "sample": f"rows {a}..{a + rows - 1} of the 4096x4096 FP64 frame (4 spp, maxIter 5000), "
                       f"{it} iterations per run, median of 3 runs ({', '.join(f'{d:.2f}' for d in dts)} s); "
-                      f"g++ -O2 -ffp-contract=off, OpenMP {threads} threads"}
+                      f"g++ -O2 -ffp-contract=off, OpenMP {threads} threads",
+            **({"vs_gpu_frame": check} if check else {})}
 
 
 def c1_leg(cu, engine, cpu: bool):
@@ -489,6 +502,7 @@ def main():
         e2e_pin.append((t2 - t0) * 1e3)
         copy_pin.append((t2 - t1) * 1e3)
     pinned.close()
+    frame_counts, frame_smooth = run.frame.counts(), run.frame.smooth()
 
     def readback_of(e2e, cp):
         return {"frame_ms_p50": round(float(np.median(e2e)), 3),
@@ -573,8 +587,9 @@ def main():
     }
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, bounded sample)")
-        result["cpu_baseline"] = cpu_baseline(cu, tile)
-        log(f"[bench] CPU {result['cpu_baseline']['value']} G-iter/s")
+        result["cpu_baseline"] = cpu_baseline(cu, tile, frame_counts, frame_smooth)
+        log(f"[bench] CPU {result['cpu_baseline']['value']} G-iter/s, band vs GPU frame "
+            f"{result['cpu_baseline'].get('vs_gpu_frame')}")
     if args.calibrate_copy_mib:
         n = args.calibrate_copy_mib << 20
         src, dst = cu.DeviceBuffer(n), cu.DeviceBuffer(n)

@@ -157,7 +157,7 @@ class ServeConfig(C.Structure):
     """cu_serve_config"""
     _fields_ = [("hbm_budget", C.c_uint64), ("deep_width", C.c_double), ("slots", C.c_uint32),
                 ("deep_method", C.c_uint32), ("max_super_sampling", C.c_float),
-                ("reserved", C.c_uint32)]
+                ("shallow_policy", C.c_uint32)]
 
 
 class FrameRequest(C.Structure):
@@ -170,7 +170,8 @@ class FrameDone(C.Structure):
     """cu_frame_done"""
     _fields_ = [("tag", C.c_uint64), ("session", C.c_uint32), ("path", C.c_uint32),
                 ("submit_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double),
-                ("iterations", C.c_ulonglong), ("checksum", C.c_ulonglong)]
+                ("iterations", C.c_ulonglong), ("checksum", C.c_ulonglong),
+                ("status", C.c_int32), ("reserved", C.c_uint32)]
 
 
 class ServeReport(C.Structure):
@@ -181,11 +182,12 @@ class ServeReport(C.Structure):
 
 
 assert C.sizeof(ServeConfig) == 32 and C.sizeof(FrameRequest) == 40
-assert C.sizeof(FrameDone) == 56 and C.sizeof(ServeReport) == 40
+assert C.sizeof(FrameDone) == 64 and C.sizeof(ServeReport) == 40
 
 DEEP_DIRECT_F64 = 1
 DEEP_PERTURB_F32 = 32
 DEEP_PERTURB_F64 = 64
+SERVE_SHALLOW_FP64, SERVE_SHALLOW_REFERENCE_SWITCH = 0, 1
 DEEP_REBASE = 1
 DEEP_REUSE_ORBIT = 2
 
@@ -285,6 +287,7 @@ SIGNATURES = {
     "cu_serve_close": (_I, [_P, _U32]),
     "cu_serve_poll": (_I, [_P, C.POINTER(FrameDone), _U32, C.POINTER(_U32), _I]),
     "cu_serve_read_rgba": (_I, [_P, _U32, _P, _SZ]),
+    "cu_serve_last_failure": (_I, [_P, C.c_char_p, _SZ]),
     "cu_serve_stats": (_I, [_P, C.POINTER(ServeReport)]),
     "cu_serve_session_times": (_I, [_P, _U32, C.POINTER(_D), C.POINTER(_D)]),
 }

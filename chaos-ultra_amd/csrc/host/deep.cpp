@@ -94,6 +94,8 @@ void free_deep(cu_deep d) {
     delete d;
 }
 
+}  // namespace
+
 void validate_deep(const cu_deep_desc &d, int method) {
     if (method != CU_DEEP_DIRECT_F64 && method != CU_DEEP_PERTURB_F32 &&
         method != CU_DEEP_PERTURB_F64)
@@ -119,6 +121,8 @@ void validate_deep(const cu_deep_desc &d, int method) {
     if (d.flags & ~(CU_DEEP_REBASE | CU_DEEP_REUSE_ORBIT))
         fail(CU_ERR_ILLEGAL_ARGUMENT, "unknown flags");
 }
+
+namespace {
 
 const char *orbit_kernel(const cu_deep_s *d) {
     return d->method == CU_DEEP_PERTURB_F32 ? "referenceOrbitFloat" : "referenceOrbitDouble";

@@ -18,6 +18,8 @@ const char *real_name(int precision);
 bool fused_safe(const cu_tile_desc &t, int precision);
 void validate_tile(const cu_tile_desc &t);
 void validate_tune(const cu_tune &tn);
+// the checks cu_deep_create runs on a deep-frame description (CU_ERR_ILLEGAL_ARGUMENT)
+void validate_deep(const cu_deep_desc &d, int method);
 bool pick_fused(int variant, const cu_tile_desc &t, int precision);
 void *device_alloc(size_t bytes);  // hipMalloc, nullptr for 0 bytes; throws CuError
 // fill `words` u32 at p with `value` by the fillU32 kernel (safe under graph capture)

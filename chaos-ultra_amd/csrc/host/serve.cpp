@@ -22,7 +22,13 @@
 //     view centre (fractalRenderMain for a session's first frame, or after a
 //     deep frame), then compose; a deeper frame runs the perturbation form;
 //   * a frame's completion is an event on its stream; its checksum (of the
-//     RGBA image) comes back through pinned host memory with it.
+//     RGBA image) comes back through pinned host memory with it;
+//   * a request that passed cu_serve_submit's checks but fails later (a device
+//     allocation at admission, a launch) completes with a non-zero status: a
+//     failure belongs to its request and never to another session's call;
+//   * cu_serve_poll takes the server's lock only for each round of retire /
+//     admit / dispatch, so a submit from another thread waits for one round at
+//     most, not for the poll's whole wait.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -100,6 +106,8 @@ struct cu_serve_s {
     size_t used = 0, peak = 0;
     unsigned long long deep_iterations = 0;
     uint64_t frames_done = 0;
+    std::deque<cu_frame_done> failed;  // completions of failed requests, reported by poll
+    std::string last_failure;
     std::chrono::steady_clock::time_point t0;
     std::mutex mutex;
 };
@@ -123,6 +131,28 @@ size_t session_bytes(uint32_t w, uint32_t h, uint32_t palette_len, uint32_t deep
     return b;
 }
 
+// The deep-frame description of request r on session ss (bailout 4, rebasing,
+// the orbit kept while the centre stays).
+cu_deep_desc deep_desc(const Session &ss, const cu_frame_request &r) {
+    const double pw = r.width / ss.width;
+    return cu_deep_desc{r.centre_x, r.centre_y, 0, 0, pw, pw, 4.0, ss.width, ss.height, r.max_iter,
+                        CU_DEEP_REBASE | CU_DEEP_REUSE_ORBIT, 0, 1e-6f};
+}
+
+// Complete request rq of session ss as failed with `status`.
+void fail_request(cu_serve_s *s, const Session &ss, const Request &rq, int status,
+                  const std::string &message) {
+    cu_frame_done d{};
+    d.tag = rq.req.tag;
+    d.session = ss.id;
+    d.path = rq.req.width < s->cfg.deep_width ? 2u : (ss.has_prev ? 0u : 1u);
+    d.submit_s = rq.submit_s;
+    d.start_s = d.done_s = now_s(s);
+    d.status = status;
+    s->failed.push_back(d);
+    s->last_failure = message;
+}
+
 void release(cu_serve_s *s, Session &ss) {
     for (void *p : {ss.buf[0], ss.buf[1], ss.rgba, ss.palette})
         if (p) (void)hipFree(p);
@@ -133,30 +163,43 @@ void release(cu_serve_s *s, Session &ss) {
     ss.state = 3;
 }
 
+// Allocate a waiting session's buffers and reserve its footprint (throws CuError).
+void admit_one(cu_serve_s *s, Session &ss) {
+    for (int k = 0; k < 2; ++k) {
+        ss.pitch[k] = pitch_of(ss.width);
+        ss.buf[k] = device_alloc(ss.pitch[k] * ss.height);
+        check(hipMemset(ss.buf[k], 0, ss.pitch[k] * ss.height), "hipMemset");
+    }
+    ss.rgba = device_alloc((size_t)ss.width * ss.height * 4);
+    ss.palette = device_alloc(ss.host_palette.size() * 4);
+    check(hipMemcpy(ss.palette, ss.host_palette.data(), ss.host_palette.size() * 4,
+                    hipMemcpyHostToDevice), "hipMemcpy H2D");
+    if (ss.deep_max_iter) {
+        const double pw = 1e-13 / ss.width;
+        cu_deep_desc d{-0.75, 0.0, 0, 0, pw, pw, 4.0, ss.width, ss.height, ss.deep_max_iter,
+                       CU_DEEP_REBASE | CU_DEEP_REUSE_ORBIT, 0, 1e-6f};
+        if (cu_deep_create(s->engine, &d, (int)s->cfg.deep_method, &ss.deep) != CU_OK)
+            fail(CU_ERR_DEVICE, cu_last_error());
+    }
+    ss.state = 1;
+    ss.admit_s = now_s(s);
+    s->used += ss.bytes;
+    s->peak = std::max(s->peak, s->used);
+}
+
+// Admit waiting sessions in order while they fit the budget.  A session whose
+// allocation fails is closed and its queued requests complete as failed.
 void admit(cu_serve_s *s) {
     for (Session &ss : s->sessions) {
         if (ss.state != 0) continue;
         if (s->used + ss.bytes > s->cfg.hbm_budget) break;  // first come first served
-        for (int k = 0; k < 2; ++k) {
-            ss.pitch[k] = pitch_of(ss.width);
-            ss.buf[k] = device_alloc(ss.pitch[k] * ss.height);
-            check(hipMemset(ss.buf[k], 0, ss.pitch[k] * ss.height), "hipMemset");
+        try {
+            admit_one(s, ss);
+        } catch (const CuError &e) {
+            release(s, ss);  // frees what was allocated; state 0 -> 3, nothing was reserved
+            for (const Request &rq : ss.queue) fail_request(s, ss, rq, e.status, e.message);
+            ss.queue.clear();
         }
-        ss.rgba = device_alloc((size_t)ss.width * ss.height * 4);
-        ss.palette = device_alloc(ss.host_palette.size() * 4);
-        check(hipMemcpy(ss.palette, ss.host_palette.data(), ss.host_palette.size() * 4,
-                        hipMemcpyHostToDevice), "hipMemcpy H2D");
-        if (ss.deep_max_iter) {
-            const double pw = 1e-13 / ss.width;
-            cu_deep_desc d{-0.75, 0.0, 0, 0, pw, pw, 4.0, ss.width, ss.height, ss.deep_max_iter,
-                           CU_DEEP_REBASE | CU_DEEP_REUSE_ORBIT, 0, 1e-6f};
-            if (cu_deep_create(s->engine, &d, (int)s->cfg.deep_method, &ss.deep) != CU_OK)
-                fail(CU_ERR_DEVICE, cu_last_error());
-        }
-        ss.state = 1;
-        ss.admit_s = now_s(s);
-        s->used += ss.bytes;
-        s->peak = std::max(s->peak, s->used);
     }
 }
 
@@ -177,11 +220,7 @@ void dispatch(cu_serve_s *s, Session &ss, Slot &sl, const Request &rq) {
     uint32_t path = 0;
     if (r.width < s->cfg.deep_width) {
         if (!ss.deep) fail(CU_ERR_ILLEGAL_ARGUMENT, "session opened without deep frames");
-        if (r.max_iter > ss.deep_max_iter)
-            fail(CU_ERR_ILLEGAL_ARGUMENT, "max_iter above the session's deep_max_iter");
-        const double pw = r.width / W;
-        cu_deep_desc d{r.centre_x, r.centre_y, 0, 0, pw, pw, 4.0, W, H, r.max_iter,
-                       CU_DEEP_REBASE | CU_DEEP_REUSE_ORBIT, 0, 1e-6f};
+        const cu_deep_desc d = deep_desc(ss, r);
         if (cu_deep_retarget(ss.deep, &d) != CU_OK) fail(CU_ERR_ILLEGAL_ARGUMENT, cu_last_error());
         if (cu_deep_render(ss.deep, (const uint32_t *)ss.palette, (uint32_t)ss.host_palette.size(),
                            (uint32_t *)ss.rgba, &sl.wrap) != CU_OK)
@@ -190,7 +229,8 @@ void dispatch(cu_serve_s *s, Session &ss, Slot &sl, const Request &rq) {
         path = 2;
     } else {
         int precision = CU_PRECISION_DOUBLE;
-        if (cu_precision_for_segment(rect.lbx, rect.lby, rect.rtx, rect.rty, (int)W, (int)H,
+        if (s->cfg.shallow_policy == CU_SERVE_SHALLOW_REFERENCE_SWITCH &&
+            cu_precision_for_segment(rect.lbx, rect.lby, rect.rtx, rect.rty, (int)W, (int)H,
                                      &precision) != CU_OK)
             fail(CU_ERR_ILLEGAL_ARGUMENT, cu_last_error());
         const bool fp32 = precision == CU_PRECISION_SINGLE;
@@ -263,21 +303,37 @@ void dispatch(cu_serve_s *s, Session &ss, Slot &sl, const Request &rq) {
     ss.busy = true;
 }
 
-// Retire finished slots into out[]; returns how many were written.
+// Retire failed requests and finished slots into out[]; returns how many were
+// written.  Never throws: a frame whose completion reports an error is returned
+// with that status.
 uint32_t retire(cu_serve_s *s, cu_frame_done *out, uint32_t capacity, uint32_t n) {
+    while (!s->failed.empty() && n < capacity) {
+        out[n++] = s->failed.front();
+        s->failed.pop_front();
+    }
     for (Slot &sl : s->slots) {
         if (!sl.busy || n >= capacity) continue;
         hipError_t q = hipEventQuery(sl.done);
         if (q == hipErrorNotReady) continue;
-        check(q, "hipEventQuery");
         Session &ss = s->sessions[sl.session - 1];
         sl.info.done_s = now_s(s);
-        sl.info.checksum = *sl.h_sum;
-        if (sl.info.path == 2) {
-            cu_deep_report rep{};
-            if (cu_deep_stats(ss.deep, &rep) != CU_OK) fail(CU_ERR_DEVICE, cu_last_error());
-            sl.info.iterations = rep.iterations;
-            s->deep_iterations += rep.iterations;
+        if (q != hipSuccess) {
+            (void)hipGetLastError();
+            sl.info.status = CU_ERR_DEVICE;
+            s->last_failure = std::string("frame: ") + hipGetErrorString(q);
+            ss.has_prev = false;
+        } else {
+            sl.info.checksum = *sl.h_sum;
+            if (sl.info.path == 2) {
+                cu_deep_report rep{};
+                if (cu_deep_stats(ss.deep, &rep) != CU_OK) {
+                    sl.info.status = CU_ERR_DEVICE;
+                    s->last_failure = cu_last_error();
+                } else {
+                    sl.info.iterations = rep.iterations;
+                    s->deep_iterations += rep.iterations;
+                }
+            }
         }
         out[n++] = sl.info;
         sl.busy = false;
@@ -300,7 +356,15 @@ void dispatch_ready(cu_serve_s *s) {
         if (!best) return;
         const Request rq = best->queue.front();
         best->queue.pop_front();
-        dispatch(s, *best, sl, rq);
+        try {
+            dispatch(s, *best, sl, rq);
+        } catch (const CuError &e) {
+            // the session's buffers may hold part of this frame: its next frame starts afresh
+            best->has_prev = false;
+            best->busy = false;
+            sl.busy = false;
+            fail_request(s, *best, rq, e.status, e.message);
+        }
     }
 }
 
@@ -349,6 +413,9 @@ int cu_serve_create(const char *kernels_dir, const cu_serve_config *cfg, cu_serv
         if (!(cfg->max_super_sampling >= 1.0f && cfg->max_super_sampling <= 64.0f))
             fail(CU_ERR_ILLEGAL_ARGUMENT, "max_super_sampling must be in [1, 64]");
         if (!(cfg->deep_width >= 0.0)) fail(CU_ERR_ILLEGAL_ARGUMENT, "deep_width must be >= 0");
+        if (cfg->shallow_policy != CU_SERVE_SHALLOW_FP64 &&
+            cfg->shallow_policy != CU_SERVE_SHALLOW_REFERENCE_SWITCH)
+            fail(CU_ERR_ILLEGAL_ARGUMENT, "unknown shallow_policy");
         s = new cu_serve_s();
         s->cfg = *cfg;
         s->t0 = std::chrono::steady_clock::now();
@@ -424,9 +491,16 @@ int cu_serve_submit(cu_serve s, uint32_t session, const cu_frame_request *req) {
         if (!(req->width > 0) || !std::isfinite(req->width) || !std::isfinite(req->centre_x) ||
             !std::isfinite(req->centre_y) || req->max_iter == 0)
             fail(CU_ERR_ILLEGAL_ARGUMENT, "bad frame request");
-        if (req->width < s->cfg.deep_width && req->max_iter > ss.deep_max_iter)
-            fail(CU_ERR_ILLEGAL_ARGUMENT,
-                 "deep frame with max_iter above the session's deep_max_iter");
+        if (req->width < s->cfg.deep_width) {
+            if (ss.deep_max_iter == 0)
+                fail(CU_ERR_ILLEGAL_ARGUMENT, "deep frame on a session opened without deep frames");
+            if (req->max_iter > ss.deep_max_iter)
+                fail(CU_ERR_ILLEGAL_ARGUMENT,
+                     "deep frame with max_iter above the session's deep_max_iter");
+            // the checks cu_deep_retarget would run at dispatch (frame within |c| <= 2,
+            // positive pixel size), so a bad request is refused here and not later
+            validate_deep(deep_desc(ss, *req), (int)s->cfg.deep_method);
+        }
         ss.queue.push_back(Request{*req, now_s(s)});
         ensure_device();
         dispatch_ready(s);
@@ -456,19 +530,31 @@ int cu_serve_poll(cu_serve s, cu_frame_done *out, uint32_t capacity, uint32_t *c
                   int wait_ms) {
     return guarded([&] {
         if (!s || !count || (capacity && !out)) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
-        std::lock_guard<std::mutex> lock(s->mutex);
         ensure_device();
         const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(wait_ms);
         uint32_t n = 0;
+        std::unique_lock<std::mutex> lock(s->mutex);
         for (;;) {
             n = retire(s, out, capacity, n);
             admit(s);
             dispatch_ready(s);
             if (n > 0 || wait_ms <= 0 || !any_busy(s) || n >= capacity) break;
             if (std::chrono::steady_clock::now() >= deadline) break;
+            lock.unlock();  // let other threads submit or close between the rounds
             std::this_thread::yield();
+            lock.lock();
         }
         *count = n;
+    });
+}
+
+int cu_serve_last_failure(cu_serve s, char *buf, size_t buf_len) {
+    return guarded([&] {
+        if (!s || !buf || buf_len == 0) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        std::lock_guard<std::mutex> lock(s->mutex);
+        const size_t n = std::min(buf_len - 1, s->last_failure.size());
+        std::memcpy(buf, s->last_failure.data(), n);
+        buf[n] = 0;
     });
 }
 

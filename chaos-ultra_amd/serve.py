@@ -37,8 +37,9 @@ def max_iter_for(width: float) -> int:
 class Server:
     def __init__(self, kernels_dir: str, hbm_budget: int, slots: int = 4,
                  deep_method: int = abi.DEEP_PERTURB_F32, max_super_sampling: float = 2.0,
-                 deep_width: float = DEEP_WIDTH):
-        self.cfg = ServeConfig(int(hbm_budget), deep_width, slots, deep_method, max_super_sampling, 0)
+                 deep_width: float = DEEP_WIDTH, shallow_policy: int = abi.SERVE_SHALLOW_FP64):
+        self.cfg = ServeConfig(int(hbm_budget), deep_width, slots, deep_method, max_super_sampling,
+                               shallow_policy)
         self.handle = C.c_void_p()
         abi.call("cu_serve_create", kernels_dir.encode(), C.byref(self.cfg), C.byref(self.handle))
         self._buf = (FrameDone * 64)()
@@ -61,6 +62,11 @@ class Server:
         n = C.c_uint32(0)
         abi.call("cu_serve_poll", self.handle, self._buf, len(self._buf), C.byref(n), wait_ms)
         return [FrameDone.from_buffer_copy(self._buf[k]) for k in range(n.value)]
+
+    def last_failure(self) -> str:
+        buf = C.create_string_buffer(512)
+        abi.call("cu_serve_last_failure", self.handle, buf, len(buf))
+        return buf.value.decode()
 
     def read_rgba(self, session: int, width: int, height: int) -> np.ndarray:
         out = np.empty(width * height, np.uint32)
@@ -143,6 +149,9 @@ def run(server: Server, sessions, realtime=True, palette=None, timeout_s=120.0):
     def take(done):
         for d in done:
             si, fi = divmod(int(d.tag), 1 << 20)
+            if d.status != 0:
+                raise RuntimeError(f"frame {fi} of session {si} failed ({d.status}): "
+                                   f"{server.last_failure()}")
             records[(si, fi)] = {"session": si, "frame": fi, "latency_s": d.done_s - d.submit_s,
                                  "queue_s": d.start_s - d.submit_s, "path": d.path,
                                  "checksum": int(d.checksum), "iterations": int(d.iterations),

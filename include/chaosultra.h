@@ -449,16 +449,23 @@ int cu_deep_orbit(cu_deep d, void *host_out, uint32_t capacity_entries, uint32_t
  * run in order.  A frame with width >= deep_width runs the mandelbrot module's
  * fractalRenderAdvanced kernel (reuse + foveation around the view centre,
  * zooming in; fractalRenderMain for a first frame) and compose
- * (cudarenderer/CudaFractalRenderer.java:158-206); a narrower one runs the
- * deep-zoom form (bailout 4, rebasing).  Replaces the single-session display
- * loop of rendering/GLRenderer.java:113-162. */
+ * (cudarenderer/CudaFractalRenderer.java:158-206), in FP64 unless
+ * shallow_policy asks for the reference's float/double switch; a narrower one
+ * runs the deep-zoom form (bailout 4, rebasing).  Replaces the single-session
+ * display loop of rendering/GLRenderer.java:113-162.
+ * A request that passes cu_serve_submit's checks but fails later (device
+ * memory, a launch) completes with a non-zero cu_frame_done.status instead of
+ * failing another call. */
+enum { CU_SERVE_SHALLOW_FP64 = 0, CU_SERVE_SHALLOW_REFERENCE_SWITCH = 1 };
 typedef struct cu_serve_config {
     uint64_t hbm_budget;       /* bytes for admitted sessions */
     double deep_width;         /* frames narrower than this use perturbation (e.g. 1e-12) */
     uint32_t slots;            /* frames in flight, one HIP stream each: 1..16 */
     uint32_t deep_method;      /* CU_DEEP_PERTURB_F32 or CU_DEEP_PERTURB_F64 */
     float max_super_sampling;  /* maxSuperSampling of fresh pixels, 1..64 */
-    uint32_t reserved;
+    uint32_t shallow_policy;   /* CU_SERVE_SHALLOW_FP64 (default): the module kernels in FP64;
+                                  CU_SERVE_SHALLOW_REFERENCE_SWITCH: FP32 unless the pixel is
+                                  below the float ulp (cudarenderer/RenderingKernel.java:124-138) */
 } cu_serve_config;
 typedef struct cu_frame_request {
     double centre_x, centre_y; /* view centre */
@@ -474,6 +481,9 @@ typedef struct cu_frame_done {
     double submit_s, start_s, done_s; /* seconds since cu_serve_create */
     unsigned long long iterations;    /* deep frames (module-kernel frames count in the total) */
     unsigned long long checksum;      /* sum of rgba[i] * (2i + 1) mod 2^64 of the frame */
+    int32_t status;                   /* CU_OK, or the CU_ERR_* the frame failed with (its
+                                         message: cu_serve_last_failure) */
+    uint32_t reserved;
 } cu_frame_done;
 typedef struct cu_serve_report {
     unsigned long long iterations;    /* every frame so far */
@@ -487,13 +497,20 @@ int cu_serve_destroy(cu_serve s);
 /* deep_max_iter: largest max_iter of the session's deep frames (0: no deep frames). */
 int cu_serve_open(cu_serve s, uint32_t width, uint32_t height, const uint32_t *palette,
                   uint32_t palette_length, uint32_t deep_max_iter, uint32_t *session);
+/* Checks the request (a finite positive width, max_iter > 0; a deep frame must lie
+ * within |c| <= 2, have a positive pixel size and max_iter <= deep_max_iter) and
+ * queues it; CU_ERR_ILLEGAL_ARGUMENT and nothing queued when a check fails. */
 int cu_serve_submit(cu_serve s, uint32_t session, const cu_frame_request *req);
 /* Release the session's memory once its queued frames are done. */
 int cu_serve_close(cu_serve s, uint32_t session);
 /* Progress: retire finished frames into out[], admit waiting sessions, start
- * ready frames.  Waits up to wait_ms for a completion while frames are in flight. */
+ * ready frames.  Waits up to wait_ms for a completion while frames are in flight,
+ * releasing the server's lock between its checks so other threads can submit
+ * meanwhile. */
 int cu_serve_poll(cu_serve s, cu_frame_done *out, uint32_t capacity, uint32_t *count,
                   int wait_ms);
+/* Message of the last frame that completed with a non-zero status ("" if none). */
+int cu_serve_last_failure(cu_serve s, char *buf, size_t buf_len);
 /* The session's last completed image (no frame may be in flight). */
 int cu_serve_read_rgba(cu_serve s, uint32_t session, uint32_t *dst, size_t count);
 int cu_serve_stats(cu_serve s, cu_serve_report *out);

@@ -8,10 +8,15 @@
 * every session's frames are the same alone and in the mix (per-frame image
   checksums), with several frames in flight at once;
 * a small HBM budget admits sessions one at a time; all frames still complete,
-  with the same results, and the peak reservation stays within the budget.
+  with the same results, and the peak reservation stays within the budget;
+* shallow frames run in FP64 unless the reference's float/double switch is
+  asked for; requests that would fail later are refused at submit; a poll that
+  waits does not hold off a submit from another thread.
 """
 import importlib
 import math
+import threading
+import time
 
 import numpy as np
 import pytest
@@ -57,22 +62,30 @@ def session_bytes(s):
     return b
 
 
-def test_frames_match_oracle_chain(gpu):
+@pytest.mark.parametrize("policy", [0, 1])
+def test_frames_match_oracle_chain(gpu, policy):
+    """policy 0: every shallow frame in FP64 (the serving configuration's rule);
+    policy 1: the reference's switch (FP32 unless the pixel is below the float ulp)."""
     cu = gpu
     W, H, max_ss = 160, 90, 2.0
-    server = sv.Server(cu.KERNELS_DIR, 1 << 30, slots=2, max_super_sampling=max_ss)
+    server = sv.Server(cu.KERNELS_DIR, 1 << 30, slots=2, max_super_sampling=max_ss,
+                       shallow_policy=policy)
     pal = sv.palette256()
     try:
         sid = server.open(W, H, pal, 0)
         centre = (-0.7436438870, 0.1318259042)
         widths = [0.02 * 0.97 ** k for k in range(3)]
+        # the float ulp at the view is far below its pixel: the switch picks FP32 there
+        assert cu.precision_for_segment(*rect(centre, widths[0], W, H), W, H) == cu.PRECISION_SINGLE
         prev = None
         for k, w in enumerate(widths):
             seg = rect(centre, w, W, H)
             server.submit(sid, centre, w, 600, k)
             done = wait_frame(server)[0]
             got = server.read_rgba(sid, W, H)
-            precision = 64 if cu.precision_for_segment(*seg, W, H) == cu.PRECISION_DOUBLE else 32
+            precision = 64
+            if policy == 1 and cu.precision_for_segment(*seg, W, H) != cu.PRECISION_DOUBLE:
+                precision = 32
             if k == 0:
                 assert done.path == 1
                 pix = O.render_main(precision, 0, (0.0, 0.0), W, H, seg, 600, max_ss, 1)
@@ -161,8 +174,19 @@ def test_serve_rejects(gpu):
         with pytest.raises(cu.IllegalArgumentError, match="budget"):
             server.open(3840, 2160, sv.palette256(), 0)
         sid = server.open(64, 36, sv.palette256(), 0)
-        with pytest.raises(cu.IllegalArgumentError, match="deep_max_iter"):
+        with pytest.raises(cu.IllegalArgumentError, match="deep frames"):
             server.submit(sid, (-0.75, 0.1), 1e-13, 1000, 0)
+        dsid = server.open(64, 36, sv.palette256(), 1000)
+        with pytest.raises(cu.IllegalArgumentError, match="deep_max_iter"):
+            server.submit(dsid, (-0.75, 0.1), 1e-13, 2000, 0)
+        # a deep frame outside |c| <= 2 is refused at submit, and nothing is queued
+        with pytest.raises(cu.IllegalArgumentError, match=r"\|c\| <= 2"):
+            server.submit(dsid, (1.9, 0.9), 1e-13, 1000, 1)
+        assert server.poll(wait_ms=0) == []
+        server.submit(dsid, (-0.75, 0.1), 1e-13, 1000, 2)
+        done = wait_frame(server)[0]
+        assert done.tag == 2 and done.status == 0
+        server.close_session(dsid)
         server.close_session(sid)
         with pytest.raises(cu.IllegalStateError):
             server.submit(sid, (-0.75, 0.1), 1e-3, 1000, 0)
@@ -196,3 +220,75 @@ def test_deep_frames_with_fp64_deltas(gpu):
         server.close()
     with pytest.raises(cu.IllegalArgumentError, match="deep_method"):
         sv.Server(cu.KERNELS_DIR, 1 << 30, slots=1, deep_method=cu.DEEP_DIRECT_F64)
+
+
+def test_poll_does_not_block_submit(gpu):
+    """A thread waiting in poll for a long frame leaves the lock free between its
+    checks: another thread's submit returns long before that poll does."""
+    cu = gpu
+    server = sv.Server(cu.KERNELS_DIR, 1 << 32, slots=1, max_super_sampling=16.0)
+    pal = sv.palette256()
+    try:
+        big = server.open(3840, 2160, pal, 0)
+        small = server.open(64, 36, pal, 0)
+        times = {}
+
+        def poller():
+            t = time.perf_counter()
+            got = []
+            while not got:
+                got = server.poll(wait_ms=5000)
+            times["poll"] = (t, time.perf_counter(), [d.tag for d in got])
+
+        server.submit(big, (-0.743643887037151, 0.131825904205330), 1e-4, 30000, 1)
+        th = threading.Thread(target=poller)
+        th.start()
+        time.sleep(0.005)
+        t = time.perf_counter()
+        server.submit(small, (-0.5, 0.0), 3.0, 100, 2)
+        times["submit"] = (t, time.perf_counter())
+        th.join(60)
+        assert not th.is_alive()
+        p0, p1, tags = times["poll"]
+        s0, s1 = times["submit"]
+        assert tags[0] == 1
+        assert p1 - p0 > 0.03, "the long frame finished too early to show anything"
+        assert s1 - s0 < 0.01 and s1 < p1 - 0.01
+        got = []
+        while len(got) < 1:
+            got += server.poll(wait_ms=100)
+        assert got[0].tag == 2 and got[0].status == 0
+    finally:
+        server.close()
+
+
+def test_full_hd_sessions_poisson_arrivals(gpu):
+    """A 1920x1080 and a 1280x720 session with Poisson arrivals at 30 frames/s,
+    replayed in real time: every frame completes, the same alone and in the mix."""
+    cu = gpu
+    ss = sv.zoom_sessions(2, 12, seed=7, resolutions=((1920, 1080),))
+    ss[1]["width"], ss[1]["height"] = 1280, 720
+    for s in ss:
+        s["widths"] = 1e-3 * sv.ZOOM ** np.arange(12)
+        s["max_iter"] = [sv.max_iter_for(w) for w in s["widths"]]
+    alone = {}
+    for s in ss:
+        server = sv.Server(cu.KERNELS_DIR, 1 << 32, slots=4)
+        try:
+            recs, _ = sv.run(server, [s], realtime=True)
+            alone[s["id"]] = [r["checksum"] for r in recs]
+        finally:
+            server.close()
+    server = sv.Server(cu.KERNELS_DIR, 1 << 32, slots=4)
+    try:
+        recs, span = sv.run(server, ss, realtime=True)
+    finally:
+        server.close()
+    mixed = {}
+    for r in recs:
+        mixed.setdefault(ss[r["session"]]["id"], []).append(r["checksum"])
+    assert alone == mixed
+    assert len(recs) == 24 and {r["path"] for r in recs} == {0, 1}
+    lat = np.array([r["latency_s"] for r in recs])
+    print(f"1080p+720p Poisson: p50 {np.percentile(lat, 50) * 1e3:.2f} ms, "
+          f"p99 {np.percentile(lat, 99) * 1e3:.2f} ms")
