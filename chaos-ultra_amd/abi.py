@@ -188,6 +188,7 @@ DEEP_DIRECT_F64 = 1
 DEEP_PERTURB_F32 = 32
 DEEP_PERTURB_F64 = 64
 SERVE_SHALLOW_FP64, SERVE_SHALLOW_REFERENCE_SWITCH = 0, 1
+DEEP_STATIC_MAX_ITER = 65535
 DEEP_REBASE = 1
 DEEP_REUSE_ORBIT = 2
 
@@ -280,6 +281,7 @@ SIGNATURES = {
     "cu_deep_outputs": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "cu_deep_stats": (_I, [_P, C.POINTER(DeepReport)]),
     "cu_deep_orbit": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
+    "cu_deep_render_static": (_I, [_I, C.POINTER(DeepDesc), _P, _P, _P, _U32, _P, _P, _P]),
     "cu_serve_create": (_I, [C.c_char_p, C.POINTER(ServeConfig), C.POINTER(_P)]),
     "cu_serve_destroy": (_I, [_P]),
     "cu_serve_open": (_I, [_P, _U32, _U32, _P, _U32, _U32, C.POINTER(_U32)]),

@@ -4,7 +4,8 @@ Outputs (git-ignored, shipped with the tree):
   kernels/<fractal>.hsaco       one module per fractal (render_generic.hip + fractals/<fractal>.hip),
                                 the counterpart of src/main/cuda/compile.sh's one PTX per fractal
   kernels/escape_family.hsaco   planar / persistent escape-time kernels and the smooth compose
-  lib/libchaosultra_hip.so      the C ABI (include/chaosultra.h), host C++ over the HIP runtime
+  lib/libchaosultra_hip.so      the C ABI (include/chaosultra.h), host C++ over the HIP runtime,
+                                plus the allocation-free deep render's kernels (deep_static.hip)
 
 Run as a script (`python chaos-ultra_amd/build.py`) or through __graft_entry__.build().
 """
@@ -41,6 +42,8 @@ HOST_FLAGS = [
 ]
 HOST_SOURCES = ("driver.cpp", "renderer.cpp", "engine.cpp", "frame.cpp", "deep.cpp", "serve.cpp",
                 "png_io.cpp")
+# host sources with device code built into the library itself (compiled by hipcc)
+HOST_HIP_SOURCES = ("deep_static.hip",)
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:
@@ -73,12 +76,17 @@ def build_modules(jobs: int = 4) -> list[str]:
 def build_library() -> str:
     os.makedirs(OUT_LIB, exist_ok=True)
     objs = []
-    with ThreadPoolExecutor(max_workers=len(HOST_SOURCES)) as ex:
+    with ThreadPoolExecutor(max_workers=len(HOST_SOURCES) + len(HOST_HIP_SOURCES)) as ex:
         def compile_one(src: str) -> str:
             obj = os.path.join(OUT_LIB, src.replace(".cpp", ".o"))
             _run(["g++", *HOST_FLAGS, "-c", os.path.join(HOST, src), "-o", obj])
             return obj
-        objs = list(ex.map(compile_one, HOST_SOURCES))
+
+        def compile_hip(src: str) -> str:
+            obj = os.path.join(OUT_LIB, src.replace(".hip", ".o"))
+            _run([_hipcc(), *HIP_FLAGS, "-fPIC", "-c", os.path.join(HOST, src), "-o", obj])
+            return obj
+        objs = list(ex.map(compile_one, HOST_SOURCES)) + list(ex.map(compile_hip, HOST_HIP_SOURCES))
     lib = os.path.join(OUT_LIB, "libchaosultra_hip.so")
     tmp = lib + ".tmp"
     _run(["g++", "-shared", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", "-lz",

@@ -57,5 +57,10 @@ int guarded(F &&body) {
 void ensure_device();
 int current_device();
 int compute_units();
+// Register a per-device set-up step: cu_init runs each registered step once,
+// with the device it selects current (deep_static.hip registers the address
+// lookup of its static scratch).  A step that throws is skipped; the code that
+// registered it repeats the step on first use and reports the error there.
+void on_device_init(void (*setup)(int device));
 
 }  // namespace cu

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 #include "handles.hpp"
@@ -27,6 +28,12 @@ int g_compute_units = 0;
 void set_last_error(const std::string &message) { t_last_error = message; }
 
 int current_device() { return g_device.load(); }
+
+std::vector<void (*)(int)> &device_setups() {
+    static std::vector<void (*)(int)> steps;
+    return steps;
+}
+void on_device_init(void (*setup)(int device)) { device_setups().push_back(setup); }
 int compute_units() { return g_compute_units; }
 
 void ensure_device() {
@@ -68,6 +75,12 @@ int cu_init(int device) {
               "hipDeviceGetAttribute", CU_ERR_INITIALIZATION);
         g_compute_units = cus;
         g_device.store(device);
+        for (void (*setup)(int) : device_setups()) {
+            try {
+                setup(device);
+            } catch (const CuError &) {
+            }
+        }
     });
 }
 

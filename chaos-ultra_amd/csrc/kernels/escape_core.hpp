@@ -331,6 +331,18 @@ __device__ __forceinline__ void tile_point<float>(const cu_tile_desc &t, uint px
 // Wave-level helpers (wave64).
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ uint lane_id() { return __lane_id(); }
+// The iteration counter is CU_ITER_SLOTS u64 slots; each wave adds its total
+// to one slot chosen by its global wave number, so a launch of a million waves
+// does not serialise on one address.  The caller sums the slots.  All 64 lanes
+// must be active: the wave sum is the device library's DPP reduction.
+__device__ __forceinline__ void add_iterations(unsigned long long *iters, uint64_t v) {
+    v = __ockl_wfred_add_u64(v);
+    const uint block = blockIdx.x + blockIdx.y * gridDim.x;
+    const uint wave = block * ((blockDim.x * blockDim.y + 63u) >> 6) +
+                      ((threadIdx.x + threadIdx.y * blockDim.x) >> 6);
+    if (iters && __lane_id() == 0 && v)
+        atomicAdd(iters + (wave & (CU_ITER_SLOTS - 1u)), (unsigned long long)v);
+}
 // This lane's bit of a wave-uniform lane mask (the inverse of ballot): the mask
 // becomes the execution mask itself, with no per-lane shift and compare.
 __device__ __forceinline__ bool in_mask(uint64_t mask) {

@@ -22,19 +22,6 @@ namespace {
 constexpr uint kMaxLdsTiles = 1024;
 constexpr uint kPaletteLds = 4096;
 
-// The iteration counter is CU_ITER_SLOTS u64 slots; each wave adds its total
-// to one slot chosen by its global wave number, so a launch of a million waves
-// does not serialise on one address.  The caller sums the slots.  All 64 lanes
-// must be active: the wave sum is the device library's DPP reduction.
-__device__ __forceinline__ void add_iterations(unsigned long long *iters, uint64_t v) {
-    v = __ockl_wfred_add_u64(v);
-    const uint block = blockIdx.x + blockIdx.y * gridDim.x;
-    const uint wave = block * ((blockDim.x * blockDim.y + 63u) >> 6) +
-                      ((threadIdx.x + threadIdx.y * blockDim.x) >> 6);
-    if (iters && cu::lane_id() == 0 && v)
-        atomicAdd(iters + (wave & (CU_ITER_SLOTS - 1u)), (unsigned long long)v);
-}
-
 // Start sample `sample_id` = (py * width + px) * spp + s of tile t (pixel (px, py)):
 // z0 and c for the tile's kind.
 template <class Real>
@@ -97,7 +84,7 @@ __device__ void escape_tile(const cu_tile_desc &t, double smooth_c, uint *counts
         counts[o] = count_sum / t.spp;
         if (smooth) smooth[o] = smooth_sum / (double)t.spp;
     }
-    add_iterations(iters, executed);
+    cu::add_iterations(iters, executed);
 }
 
 // ---------------------------------------------------------------------------
@@ -155,7 +142,7 @@ __device__ void escape_tile_samples(const cu_tile_desc &t, double smooth_c, uint
         counts[o] = count_sum / spp;
         if (smooth) smooth[o] = smooth_sum / (double)spp;
     }
-    add_iterations(iters, executed);
+    cu::add_iterations(iters, executed);
 }
 
 // ---------------------------------------------------------------------------
@@ -298,7 +285,7 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
             }
         }
     }
-    add_iterations(iters, executed);
+    cu::add_iterations(iters, executed);
 }
 
 }  // namespace

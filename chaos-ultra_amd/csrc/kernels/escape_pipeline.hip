@@ -156,7 +156,7 @@ __device__ __forceinline__ bool run_lanes(const PipeArgs &a, bool active, Real x
             out_queue[base + rank] = s;
         }
     }
-    add_iterations(a.iters, executed);
+    cu::add_iterations(a.iters, executed);
     return mode == 2;
 }
 
@@ -450,7 +450,7 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
             mode = 0;
         }
     }
-    add_iterations(a.iters, executed);
+    cu::add_iterations(a.iters, executed);
 }
 
 }  // namespace

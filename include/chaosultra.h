@@ -438,6 +438,26 @@ int cu_deep_stats(cu_deep d, cu_deep_report *out);
  * {Zx, Zy, |Z|^2/2, 0} in the delta precision; *length = M (entries M + 1). */
 int cu_deep_orbit(cu_deep d, void *host_out, uint32_t capacity_entries, uint32_t *length);
 
+/* ---- deep zoom without allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Renders the frame cu_deep_render renders for a cu_deep
+ * created with the same desc and method (perturbation with CU_DEEP_REBASE, so
+ * no glitch rounds) into the caller's planes: counts (W*H u32, required),
+ * smooth (W*H f64, may be NULL), RGBA8 through the palette (both or neither),
+ * iterations into CU_ITER_SLOTS zeroed u64 slots (may be NULL).  The reference
+ * orbit and its length live in a static device array of CU_DEEP_STATIC_BYTES
+ * per GPU, held until the process ends; max_iter above CU_DEEP_STATIC_MAX_ITER is
+ * CU_ERR_ILLEGAL_ARGUMENT.  The call enqueues two kernels on `stream` and
+ * allocates nothing.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams or threads) race.  CU_DEEP_REUSE_ORBIT is
+ * accepted and has no effect (the orbit is computed on every call). */
+#define CU_DEEP_STATIC_MAX_ITER 65535u
+#define CU_DEEP_STATIC_BYTES ((CU_DEEP_STATIC_MAX_ITER + 1u) * 32u + 256u)
+int cu_deep_render_static(int method, const cu_deep_desc *desc, uint32_t *d_counts,
+                          double *d_smooth, const uint32_t *d_palette, uint32_t palette_length,
+                          uint32_t *d_rgba, unsigned long long *d_iters, cu_stream stream);
+
 /* ---- serving: continuous batching of zoom-session frame requests ----
  * Many sessions, one GPU.  A session owns the reference's ping-pong
  * pixel_info_t buffers, an RGBA8 image, its palette and a deep-zoom frame, all
