@@ -21,6 +21,13 @@ struct Fractal {
 
     __device__ static float value(uint n, uint) { return (float)n; }
 
+    // julia_c (in the kernel's precision) zero or at least `threshold` in magnitude.
+    template <class Real>
+    __device__ static bool c_fused_valid(double threshold) {
+        const double re = (double)(Real)julia_c[0], im = (double)(Real)julia_c[1];
+        return (re == 0 || fabs(re) >= threshold) && (im == 0 || fabs(im) >= threshold);
+    }
+
     __device__ static bool blocks_valid(double bailout_sq) {
         return cu::blocks_valid(bailout_sq, true, julia_c[0], julia_c[1]);
     }

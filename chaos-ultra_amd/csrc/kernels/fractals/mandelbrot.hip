@@ -20,6 +20,10 @@ struct Fractal {
     // computeFractal's return value for an orbit of n of max_iter iterations.
     __device__ static float value(uint n, uint max_iter) { return n == max_iter ? 0.f : (float)n; }
 
+    // c is the plane point: the fused-step rule has nothing to add.
+    template <class Real>
+    __device__ static bool c_fused_valid(double) { return true; }
+
     __device__ static bool blocks_valid(double bailout_sq) {
         return cu::blocks_valid(bailout_sq, false, 0, 0);
     }

@@ -17,6 +17,9 @@
 // reference votes over.  No lane returns early: out-of-image lanes stay in the
 // wave with no work, so every ballot runs in wave-uniform control flow and the
 // vote membership is explicit (same half-wave, same call site, still sampling).
+// A wave's orbits are pooled across its 64 pixels and their samples (see
+// sample_the_fractal), so a lane whose orbit ends takes another instead of
+// idling until the wave's slowest pixel is done.
 #include "cu_types.hpp"
 #include "escape_core.hpp"
 
@@ -123,20 +126,96 @@ __device__ __forceinline__ float dispersion(const float (&samples)[kAdaptiveThre
     return variance / mean;
 }
 
-// sampleTheFractal<Real> (fractalRendererGeneric.cu:85-155) for the lanes with
-// site != 0; lanes sharing a half-wave and a site vote together.  Must be called
-// by the whole wave (the loop and the ballots are wave-uniform).
+// Pixel of flat thread index t of the block (getImageIndexes, :36-53).
+__device__ __forceinline__ Point<uint> image_indexes_at(uint t) {
+    const uint warp = t >> 5, lane = t & 31u;
+    Point<uint> p;
+    p.x = blockIdx.x * 32u + (warp & 3u) * 8u + (lane >> 4) * 4u + (lane & 3u);
+    p.y = blockIdx.y * 32u + (warp >> 2) * 4u + ((lane >> 2) & 3u);
+    return p;
+}
+
+// Sample offsets of sample i of a pixel asked for sampleCountF samples (:105-117).
 template <class Real>
-__device__ uint sample_the_fractal(int site, Point<uint> pixel, Point<uint> grid,
-                                   Rectangle<Real> image, uint max_iter, float &sample_count_f,
-                                   bool adaptive, uint64_t &executed) {
-    const uint64_t half = (cu::lane_id() < 32) ? 0x00000000ffffffffull : 0xffffffff00000000ull;
+__device__ __forceinline__ void sample_offset(uint i, float sample_count_f, Real &dx, Real &dy) {
+    const uint skip_first = 2;
+    if (i <= skip_first) {
+        dx = dy = (Real)i / (Real)(skip_first + 1);
+    } else {
+        const float per_row_f = sqrtf(sample_count_f - skip_first);
+        const uint per_row_i = (uint)roundf(per_row_f);
+        dx = (float)((i - skip_first) % per_row_i) / per_row_f;
+        dy = (float)((i - skip_first) / per_row_i) / per_row_f;
+    }
+}
+
+// Whether round i of a pixel taking n samples holds the votes (:133, with the
+// reference's && / || precedence).
+__device__ __forceinline__ bool vote_round(uint i, uint n, bool adaptive) {
+    return (adaptive && (i > 0 && i < kAdaptiveThreshold)) || (i == n / 2);
+}
+
+// The first round >= c (c < n) whose votes may stop the pixel: every sample up to
+// it is needed whatever the votes say.
+__device__ __forceinline__ uint next_vote_round(uint c, uint n, bool adaptive) {
+    if (adaptive && c < kAdaptiveThreshold) return c == 0 ? (n / 2 == 0 ? 0u : 1u) : c;
+    return c <= n / 2 ? n / 2 : n - 1;
+}
+
+// Rounds of results a pixel may have waiting, and the number of waiting lanes at
+// which a wave stops its unchecked blocks to hand out new samples.
+constexpr uint kRing = 4;
+constexpr uint kRefill = 16;
+constexpr unsigned long long kAbsent = ~0ull;  // above any iteration count
+
+// One wave's LDS: each pixel's results of the rounds not yet voted on
+// (iterations executed, kAbsent until posted) and the hand-out slots.
+struct WaveLds {
+    unsigned long long result[kRing][64];  // [sample % kRing][owner lane]
+    uint handout[64];        // owner lane | sample << 8, by rank
+};
+
+// Orders this wave's LDS accesses across the phases below (the LDS serves a
+// wave's instructions in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// sampleTheFractal<Real> (fractalRendererGeneric.cu:85-155) for the lanes with
+// site != 0, whole wave, with the wave's orbits pooled.
+//
+// Lane k owns the pixel of lane k: its sample count, escape sum and first 10
+// sample values, and its votes with the lanes sharing its half-wave and site
+// (the reference's 32-lane warp and call site).  Orbits are pooled apart from
+// that: a pixel's samples are independent orbits, so any lane of the wave may
+// iterate sample j of any pixel of the wave.  Lanes run unchecked blocks of K
+// steps; once kRefill lanes wait, the finished orbits post their iteration
+// counts to the owner's LDS ring, every round whose vote group has all its
+// results is voted on exactly as the round-by-round loop does, and idle lanes
+// take new samples — first those every outcome of the next vote needs, then
+// samples past it, up to kRing rounds ahead of the votes.  A sample that a vote
+// makes unnecessary is dropped (or its posted result cleared) and never added
+// to the sums or to the iteration count, so pixels, sample counts and
+// iterations equal the round-by-round loop's.  Must be called by the whole wave.
+template <class Real, bool kFused>
+__device__ uint sample_the_fractal(WaveLds &lds, int site, Point<uint> grid, Rectangle<Real> image,
+                                   uint max_iter, float &sample_count_f, bool adaptive,
+                                   uint64_t &executed) {
+    constexpr int K = (int)kBlockSteps;
+    const uint lane = cu::lane_id();
+    const uint wave_base = (threadIdx.x + threadIdx.y * blockDim.x) & ~63u;
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+    const uint64_t half = (lane < 32) ? 0x00000000ffffffffull : 0xffffffff00000000ull;
     const uint64_t site1 = cu::ballot(site == 1), site2 = cu::ballot(site == 2);
     const uint64_t group = half & (site == 1 ? site1 : (site == 2 ? site2 : 0ull));
-    const bool blocks = Fractal::blocks_valid(4.0);
+    const bool blocks = Fractal::blocks_valid(4.0) && max_iter >= (uint)K;
+    const Real bsq = (Real)4;
 
+    // ---- the pixel this lane owns ----
     bool in_loop = site != 0;
     uint sample_count = 0, escape_sum = 0, result = 0;
+    uint confirmed = 0, issued = 0;  // rounds voted on; samples handed out
     float samples[kAdaptiveThreshold];
 #pragma unroll
     for (uint j = 0; j < kAdaptiveThreshold; ++j) samples[j] = 0;
@@ -148,58 +227,186 @@ __device__ uint sample_the_fractal(int site, Point<uint> pixel, Point<uint> grid
             sample_count = min(kMaxSuperSampling, (uint)roundf(sample_count_f));
         }
     }
+    const float count_f = sample_count_f;  // the offsets use the caller's sampleCountF
+#pragma unroll
+    for (uint r = 0; r < kRing; ++r) lds.result[r][lane] = kAbsent;
     // pixelSize = image.size() / gridSize.cast<Real>()  (:97)
     const Real psx = (image.right_top.x - image.left_bottom.x) / (Real)grid.x;
     const Real psy = (image.right_top.y - image.left_bottom.y) / (Real)grid.y;
-    const uint skip_first = 2;
 
-    for (uint i = 0; __any(in_loop); ++i) {
-        if (in_loop) {
-            Real dx, dy;
-            if (i <= skip_first) {
-                dx = dy = (Real)i / (Real)(skip_first + 1);
-            } else {
-                const float per_row_f = sqrtf(sample_count_f - skip_first);
-                const uint per_row_i = (uint)roundf(per_row_f);
-                dx = (float)((i - skip_first) % per_row_i) / per_row_f;
-                dy = (float)((i - skip_first) / per_row_i) / per_row_f;
-            }
-            const Real X = image.left_bottom.x + ((Real)pixel.x + dx) * psx;
-            const Real Y = image.right_top.y - ((Real)pixel.y + dy) * psy;
-            const uint escape_time = (uint)compute_fractal<Real>(max_iter, X, Y, blocks, executed);
-            escape_sum += escape_time;
+    // ---- the orbit this lane iterates ----
+    int mode = 0;  // 0 idle, 1 unchecked blocks, 2 pending (locate the escape / checked steps)
+    bool replay = false;
+    uint owner = lane, sample = 0, i = 0;
+    Real x = 0, y = 0, cx = 0, cy = 0, sx = 0, sy = 0, mx = 0, my = 0;
+    wave_lds_fence();
+
+    for (;;) {
+        // ---- vote on every round whose group has posted all its results ----
+        for (;;) {
+            const uint slot = confirmed % kRing;
+            const unsigned long long posted = in_loop ? lds.result[slot][lane] : kAbsent;
+            const bool have = in_loop && posted != kAbsent;
+            const uint n_here = (uint)posted;
+            // every ballot runs in wave-uniform control flow (never inside && or ?:)
+            const uint64_t missing = cu::ballot(in_loop && !have);
+            const bool ready = have && (missing & group) == 0;
+            if (cu::ballot(ready) == 0) break;
+            const uint ri = confirmed;
+            if (ready) {
+                lds.result[slot][lane] = kAbsent;
+                const uint escape_time = (uint)Fractal::value(n_here, max_iter);
+                executed += n_here;
+                escape_sum += escape_time;
 #pragma unroll
-            for (uint j = 0; j < kAdaptiveThreshold; ++j)
-                if (j == i) samples[j] = (float)escape_time;
+                for (uint j = 0; j < kAdaptiveThreshold; ++j)
+                    if (j == ri) samples[j] = (float)escape_time;
+            }
+            const bool part = ready && vote_round(ri, sample_count, adaptive);
+            float disp = 0;
+            if (part) {
+                const float mean = escape_sum / (ri + 1);  // integer mean (:135)
+                disp = dispersion(samples, ri, mean);
+            }
+            const uint64_t P = cu::ballot(part) & group;
+            const bool v1 =
+                (cu::ballot(part && fabsf(samples[0] - samples[1]) < FLT_EPSILON) & group) == P;
+            const bool v2 = (cu::ballot(part && disp < 0.01f) & group) == P;
+            const bool v3 = (cu::ballot(part && disp <= 1.f) & group) == P;
+            if (part) {
+                if (ri == 1 && v1) {
+                    sample_count = ri + 1;
+                } else if (v2) {
+                    sample_count = ri + 1;
+                } else if (v3 && ri >= sample_count / 2) {
+                    sample_count = ri + 1;
+                }
+            }
+            if (ready) {
+                confirmed = ri + 1;
+                if (!(confirmed < sample_count)) {
+                    in_loop = false;
+                    result = escape_sum / sample_count;  // :152
+                    sample_count_f = (float)sample_count;
+#pragma unroll
+                    for (uint r = 0; r < kRing; ++r) lds.result[r][lane] = kAbsent;
+                }
+            }
+            wave_lds_fence();
         }
-        const bool part = in_loop && ((adaptive && (i > 0 && i < kAdaptiveThreshold)) ||
-                                      (i == sample_count / 2));
-        float disp = 0;
-        if (part) {
-            const float mean = escape_sum / (i + 1);
-            disp = dispersion(samples, i, mean);
+        // ---- drop orbits whose pixel has stopped before them ----
+        {
+            const bool owner_in = __shfl(in_loop ? 1 : 0, (int)owner, 64) != 0;
+            const uint owner_count = (uint)__shfl((int)sample_count, (int)owner, 64);
+            if (mode != 0 && !(owner_in && sample < owner_count)) mode = 0;
         }
-        const uint64_t P = cu::ballot(part) & group;
-        const bool v1 =
-            (cu::ballot(part && fabsf(samples[0] - samples[1]) < FLT_EPSILON) & group) == P;
-        const bool v2 = (cu::ballot(part && disp < 0.01f) & group) == P;
-        const bool v3 = (cu::ballot(part && disp <= 1.f) & group) == P;
-        if (part) {
-            if (i == 1 && v1) {
-                sample_count = i + 1;
-            } else if (v2) {
-                sample_count = i + 1;
-            } else if (v3 && i >= sample_count / 2) {
-                sample_count = i + 1;
+        if (cu::ballot(in_loop) == 0) break;
+
+        // ---- hand out samples: those the next vote needs, then later ones ----
+        bool more = false;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (;;) {
+                const uint64_t idle = cu::ballot(mode == 0);
+                const bool can = in_loop && issued < sample_count && issued < confirmed + kRing &&
+                                 (pass == 1 || issued <= next_vote_round(confirmed, sample_count,
+                                                                         adaptive));
+                const uint64_t avail = cu::ballot(can);
+                more = avail != 0;
+                if (idle == 0 || avail == 0) break;
+                const uint n_take = min((uint)__popcll(idle), (uint)__popcll(avail));
+                const uint orank = (uint)__popcll(avail & lanes_below);
+                if (can && orank < n_take) {
+                    lds.handout[orank] = lane | (issued << 8);
+                    ++issued;
+                }
+                wave_lds_fence();
+                const bool take = mode == 0 && (uint)__popcll(idle & lanes_below) < n_take;
+                uint h = lane;
+                if (take) h = lds.handout[__popcll(idle & lanes_below)];
+                wave_lds_fence();
+                const uint src = h & 63u;
+                const float ocount = __shfl(count_f, (int)src, 64);
+                if (take) {
+                    owner = src;
+                    sample = h >> 8;
+                    const Point<uint> op = image_indexes_at(wave_base + owner);
+                    Real dx, dy;
+                    sample_offset<Real>(sample, ocount, dx, dy);
+                    // c = leftTop + (1,-1) * (pixel + delta) * pixelSize  (:119-124)
+                    const Real X = image.left_bottom.x + ((Real)op.x + dx) * psx;
+                    const Real Y = image.right_top.y - ((Real)op.y + dy) * psy;
+                    Fractal::start<Real>(X, Y, x, y, cx, cy);
+                    i = 0;
+                    replay = false;
+                    mode = blocks ? 1 : 2;
+                }
             }
         }
-        if (in_loop && !(i + 1 < sample_count)) {
-            in_loop = false;
-            result = escape_sum / sample_count;
-            sample_count_f = (float)sample_count;
+
+        // ---- unchecked blocks until kRefill lanes wait (idle ones count while work is left) ----
+        for (;;) {
+            if (mode == 1) {
+                sx = x;
+                sy = y;
+#pragma unroll
+                for (int k = 0; k < K / 2; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+                mx = x;
+                my = y;
+#pragma unroll
+                for (int k = K / 2; k < K; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+                if (x * x + y * y < bsq) {
+                    i += K;
+                    if (max_iter - i < (uint)K) mode = 2;
+                } else {
+                    mode = 2;
+                    replay = true;
+                }
+            }
+            const uint64_t run = cu::ballot(mode == 1);
+            const uint64_t pending = cu::ballot(mode == 2);
+            const uint64_t idle = cu::ballot(mode == 0);
+            const uint waiting = (uint)__popcll(pending) + (more ? (uint)__popcll(idle) : 0u);
+            if (run == 0 || waiting >= kRefill) break;
         }
+
+        // ---- finish the pending orbits and post their iteration counts ----
+        if (mode == 2) {
+            Real mag2 = 0;
+            if (replay) {
+                // a failing test at max_iter itself is never reached by the reference loop:
+                // n == max_iter then reads as an orbit that did not escape
+                i = cu::locate_escape<Real, K, kFused>(x, y, sx, sy, mx, my, cx, cy, i, bsq, mag2);
+                replay = false;
+            } else {
+                bool escaped = false;
+                cu::checked_steps<Real, kFused>(x, y, cx, cy, i, max_iter, bsq, escaped, mag2);
+            }
+            lds.result[sample % kRing][owner] = (unsigned long long)i;
+            mode = 0;
+        }
+        wave_lds_fence();
     }
     return result;
+}
+
+// Whether the fused step (fma(x*y, 2, cy)) may replace ((2x)*y)+cy for this
+// launch: the rule the planar tiles use (the origin, pixel sizes and a Julia c
+// each zero or far from the subnormal range; in FP32 an origin of at least
+// 2^-11), so that x*y is never a nonzero subnormal and both forms round alike.
+template <class Real>
+__device__ __forceinline__ bool fused_valid(Rectangle<Real> image, Point<uint> grid) {
+    const Real left = image.left_bottom.x, top = image.right_top.y;
+    const Real psx = (image.right_top.x - image.left_bottom.x) / (Real)grid.x;
+    const Real psy = (image.right_top.y - image.left_bottom.y) / (Real)grid.y;
+    if constexpr (sizeof(Real) == 8) {
+        const double T = 0x1.0p-300;
+        return (left == 0 || fabs(left) >= T) && (top == 0 || fabs(top) >= T) && psx >= T &&
+               psy >= T && Fractal::template c_fused_valid<Real>(T);
+    } else {
+        const double T = 0x1.0p-11;
+        return fabs((double)left) >= T && fabs((double)top) >= T &&
+               Fractal::template c_fused_valid<Real>(T);
+    }
 }
 
 template <class Real>
@@ -209,8 +416,14 @@ __device__ void render_main(pixel_info_t *out, long out_pitch, Point<uint> size,
     const bool inside = idx.x < size.x && idx.y < size.y;
     float sample_count = max_ss;
     uint64_t executed = 0;
-    const uint r = sample_the_fractal<Real>(inside ? 1 : 0, idx, size, image, max_iter,
-                                            sample_count, (flags & kAdaptiveFlag) != 0, executed);
+    __shared__ WaveLds lds_waves[16];
+    WaveLds &lds = lds_waves[(threadIdx.x + threadIdx.y * blockDim.x) >> 6];
+    const bool adaptive = (flags & kAdaptiveFlag) != 0;
+    const uint r = fused_valid<Real>(image, size)
+                       ? sample_the_fractal<Real, true>(lds, inside ? 1 : 0, size, image, max_iter,
+                                                        sample_count, adaptive, executed)
+                       : sample_the_fractal<Real, false>(lds, inside ? 1 : 0, size, image, max_iter,
+                                                         sample_count, adaptive, executed);
     count_iterations(executed);
     if (inside) {
         pixel_info_t p;
@@ -337,8 +550,14 @@ __device__ void render_advanced(pixel_info_t *out, long out_pitch, Point<uint> s
         }
     }
     uint64_t executed = 0;
-    const uint r = sample_the_fractal<Real>(site, idx, size, image, max_iter, sample_count,
-                                            (flags & kAdaptiveFlag) != 0, executed);
+    __shared__ WaveLds lds_waves[16];
+    WaveLds &lds = lds_waves[(threadIdx.x + threadIdx.y * blockDim.x) >> 6];
+    const bool adaptive = (flags & kAdaptiveFlag) != 0;
+    const uint r = fused_valid<Real>(image, size)
+                       ? sample_the_fractal<Real, true>(lds, site, size, image, max_iter,
+                                                        sample_count, adaptive, executed)
+                       : sample_the_fractal<Real, false>(lds, site, size, image, max_iter,
+                                                         sample_count, adaptive, executed);
     count_iterations(executed);
     if (site == 1) {  // :349-356
         const float samples = (float)r;

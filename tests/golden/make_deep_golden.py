@@ -8,9 +8,9 @@ centre + ((px + 0.5 - 960) * pw, (540 - (py + 0.5)) * pw).
 
 The subset is numpy.random.default_rng(SEED).choice(1920*1080, 65536, replace=False),
 sorted; the fixture stores the indices, the escape counts (Mandelbrot convention:
-maxIter -> 0) and the smooth value n + 1 - log2(ln|z|^2 / ln 256) as float32.
+maxIter -> 0) and the smooth value n + 1 - log2(ln|z|^2 / ln 256) as float64.
 
-Run: python tests/golden/make_deep_golden.py   (about 15 s on 8 cores)
+Run: python tests/golden/make_deep_golden.py [out.npz]   (about 15 s on 8 cores)
 """
 from __future__ import annotations
 
@@ -57,10 +57,11 @@ def main():
                           0.0)
         key = f"{w:.0e}"
         out[f"counts_{key}"] = counts.astype(np.uint16)
-        out[f"smooth_{key}"] = smooth.astype(np.float32)
+        out[f"smooth_{key}"] = smooth.astype(np.float64)
         print(f"width {w}: mean count {counts.mean():.1f}, interior {(state == 2).mean():.3f}")
-    np.savez_compressed(OUT, **out)
-    print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    dest = sys.argv[1] if len(sys.argv) > 1 else OUT
+    np.savez_compressed(dest, **out)
+    print("wrote", dest, os.path.getsize(dest), "bytes")
 
 
 if __name__ == "__main__":

@@ -180,11 +180,21 @@ def test_deep_rejects(gpu, engine):
         f.close()
 
 
+# Share of the 65 536-pixel subset whose count differs from the double-double
+# oracle, per width: the bound each method is held to.  FP32 deltas are far less
+# accurate at this centre (measured 6.48 % at 1e-6 and 12.95 % at 1e-10, equal to
+# the oracle's FP32 perturbation, which the kernel matches bit for bit); direct
+# FP64 is held to what it measures (1.80 % and 15.21 %).
+MISMATCH_BOUND = {1e-6: {64: 0.005, 32: 0.070, 1: 0.020},
+                  1e-10: {64: 0.005, 32: 0.135, 1: 0.160},
+                  1e-20: {64: 0.005, 32: 0.005}}
+
+
 @pytest.mark.parametrize("width", [1e-6, 1e-10, 1e-20])
 def test_c4_frames_against_double_double(gpu, engine, width):
-    """1920x1080 at maxIter 50000: FP64-delta perturbation within 1 % of the
-    double-double oracle on the seeded subset (measured 0.1-0.3 %); the FP32 and
-    direct-FP64 rates are printed (FP32 deltas: see the accuracy table)."""
+    """1920x1080 at maxIter 50000: each method's share of mismatching counts on the
+    seeded subset against the double-double oracle stays within MISMATCH_BOUND
+    (FP64 deltas 0.1-0.3 %, FP32 deltas 6.5 % / 13.0 %, direct FP64 1.8 % / 15.2 %)."""
     cu = gpu
     g = np.load(os.path.join(GOLDEN, "deep_dd_subset.npz"))
     idx = g["index"]
@@ -198,7 +208,8 @@ def test_c4_frames_against_double_double(gpu, engine, width):
         rates[method] = float(np.mean(got != want))
         print(f"width {width:g} method {method}: mismatch vs double-double "
               f"{100 * rates[method]:.3f} %, {rep.iterations} iterations, {rep.total_ms:.2f} ms")
-    assert rates[64] <= 0.01
+    for method, bound in MISMATCH_BOUND[width].items():
+        assert rates[method] <= bound, (width, method, rates[method])
 
 
 @pytest.mark.parametrize("centre,width", [(CENTRE, 1e-10), ((-0.5, 0.5), 0.3)])
