@@ -262,6 +262,11 @@ SIGNATURES = {
     "cu_escape_tile": (_I, [_P, _I, _I, C.POINTER(TileDesc), _P, _P, _P, _P]),
     "cu_escape_tiles_persistent": (_I, [_P, _I, _I, _P, _U32, _U64, _P, _P, _P, _P,
                                         C.POINTER(Tune), _P]),
+    "cu_escape_tiles_persistent_capture": (_I, [_P, _I, _I, _P, _U32, _U64, _P, _P, _P, _P,
+                                                C.POINTER(Tune), C.POINTER(_P)]),
+    "cu_graph_launch": (_I, [_P, _P]),
+    "cu_graph_last_ms": (_I, [_P, C.POINTER(C.c_float)]),
+    "cu_graph_destroy": (_I, [_P]),
     "cu_compose_smooth": (_I, [_P, _P, _U32, _P, _U32, _U32, _U32, _P, _U32, _P]),
     "cu_engine_last_kernel_ms": (_I, [_P, C.POINTER(C.c_float)]),
     "cu_iters_total": (_I, [_P, C.POINTER(C.c_ulonglong)]),

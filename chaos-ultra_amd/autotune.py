@@ -120,6 +120,34 @@ class PersistentBatch:
                                        self.counts, self.smooth, self.iters, self.queue, tune)
         return engine.last_kernel_ms()
 
+    def capture(self, engine, tune, precision=64):
+        """The persistent launch (clears included) as one replayable HIP graph."""
+        g = C.c_void_p()
+        self.cu.abi.call("cu_escape_tiles_persistent_capture", engine.handle, precision, self.variant,
+                         self.d_tiles.ptr, self.n, C.c_uint64(self.total), self.counts.ptr,
+                         self.smooth.ptr, self.iters.ptr, self.queue.ptr, C.byref(tune), C.byref(g))
+        return CapturedLaunch(self.cu, g)
+
+
+class CapturedLaunch:
+    """A captured launch sequence (cu_graph)."""
+
+    def __init__(self, cu, handle):
+        self.cu, self.handle = cu, handle
+
+    def replay(self, stream=None) -> None:
+        self.cu.abi.call("cu_graph_launch", self.handle, stream.handle if stream else None)
+
+    def last_ms(self) -> float:
+        ms = C.c_float(0)
+        self.cu.abi.call("cu_graph_last_ms", self.handle, C.byref(ms))
+        return ms.value
+
+    def close(self) -> None:
+        if self.handle and self.handle.value:
+            self.cu.abi.call("cu_graph_destroy", self.handle)
+            self.handle = C.c_void_p()
+
 
 def tune_batch(cu, engine, tiles, precision=64, repeats=3, warmup=1, persistent=True, log=None):
     """Measure both engines on `tiles`; returns a dict with the winner and the tables."""

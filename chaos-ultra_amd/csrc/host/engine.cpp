@@ -105,6 +105,45 @@ bool pick_fused(int variant, const cu_tile_desc &t, int precision) {
     fail(CU_ERR_ILLEGAL_ARGUMENT, "variant must be -1, 0 or 1");
 }
 
+// Arguments of the persistent launch, checked; the tuning it runs with.
+cu_tune persistent_args(cu_engine e, int variant, const cu_tile_desc *d_tiles, uint32_t n_tiles,
+                        uint32_t *d_counts, unsigned long long *d_queue, const cu_tune *tune) {
+    if (!e || !d_tiles || !d_counts || !d_queue) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+    if (n_tiles == 0 || n_tiles > kMaxPersistentTiles)
+        fail(CU_ERR_ILLEGAL_ARGUMENT, "n_tiles must be in [1, 1024]");
+    if (variant < -1 || variant > 1) fail(CU_ERR_ILLEGAL_ARGUMENT, "variant must be -1, 0 or 1");
+    if (variant == -1)
+        fail(CU_ERR_ILLEGAL_ARGUMENT,
+             "the persistent launch takes an explicit variant (check each tile with cu_tile_fused_safe)");
+    cu_tune tn = e->tune;
+    if (tune) {
+        validate_tune(*tune);
+        tn = *tune;
+    }
+    return tn;
+}
+
+// The persistent launch on `stream`: the queue counter cleared by a kernel (safe
+// under graph capture), with clear_iters the iteration slots too, then the
+// kernel itself.
+void enqueue_persistent(cu_engine e, int precision, int variant, const cu_tile_desc *d_tiles,
+                        uint32_t n_tiles, uint64_t total, uint32_t *d_counts, double *d_smooth,
+                        unsigned long long *d_iters, unsigned long long *d_queue, const cu_tune &tn,
+                        cu_stream stream, bool clear_iters, bool timed) {
+    std::string name = std::string("escapeTilesPersistent") + real_name(precision) +
+                       (variant == 1 ? "Fused" : "Exact") + "K" + std::to_string(tn.block_steps);
+    cu_function f = function(e, name);
+    fill_u32(e, stream, d_queue, 0, 2);
+    if (clear_iters && d_iters) fill_u32(e, stream, d_iters, 0, (uint64_t)CU_ITER_SLOTS * 2);
+    const uint32_t cus = (uint32_t)std::max(1, compute_units());
+    dim3 block(256, 1, 1);
+    dim3 grid(cus * tn.waves_per_cu / 4, 1, 1);
+    uint32_t chunk = tn.chunk, refill = tn.refill_lanes;
+    void *params[] = {&d_tiles, &n_tiles, &total, &d_counts, &d_smooth, &d_iters,
+                      &d_queue, &chunk, &refill};
+    launch(e, f, grid, block, params, stream, timed);
+}
+
 }  // namespace cu
 
 using namespace cu;
@@ -213,38 +252,89 @@ int cu_escape_tiles_persistent(cu_engine e, int precision, int variant, const cu
                                double *d_smooth, unsigned long long *d_iters,
                                unsigned long long *d_queue, const cu_tune *tune, cu_stream stream) {
     return guarded([&] {
-        if (!e || !d_tiles || !d_counts || !d_queue) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
-        if (n_tiles == 0 || n_tiles > kMaxPersistentTiles)
-            fail(CU_ERR_ILLEGAL_ARGUMENT, "n_tiles must be in [1, 1024]");
-        if (variant < -1 || variant > 1) fail(CU_ERR_ILLEGAL_ARGUMENT, "variant must be -1, 0 or 1");
-        if (variant == -1)
-            fail(CU_ERR_ILLEGAL_ARGUMENT,
-                 "the persistent launch takes an explicit variant (check each tile with cu_tile_fused_safe)");
+        const cu_tune tn = persistent_args(e, variant, d_tiles, n_tiles, d_counts, d_queue, tune);
         std::lock_guard<std::mutex> lock(e->mutex);
         ensure_device();
-        cu_tune tn = e->tune;
-        if (tune) {
-            validate_tune(*tune);
-            tn = *tune;
-        }
         if (total == 0) return;
-        std::string name = std::string("escapeTilesPersistent") + real_name(precision) +
-                           (variant == 1 ? "Fused" : "Exact") + "K" + std::to_string(tn.block_steps);
-        cu_function f = function(e, name);
-        // clear the queue counter with a kernel (safe under graph capture)
-        cu_function fill = function(e, "fillU32");
-        uint32_t *q32 = (uint32_t *)d_queue;
-        uint32_t zero = 0;
-        uint64_t two = 2;
-        void *fill_params[] = {&q32, &zero, &two};
-        launch(e, fill, dim3(1), dim3(64), fill_params, stream, false);
-        const uint32_t cus = (uint32_t)std::max(1, compute_units());
-        dim3 block(256, 1, 1);
-        dim3 grid(cus * tn.waves_per_cu / 4, 1, 1);
-        uint32_t chunk = tn.chunk, refill = tn.refill_lanes;
-        void *params[] = {&d_tiles, &n_tiles, &total, &d_counts, &d_smooth, &d_iters,
-                          &d_queue, &chunk, &refill};
-        launch(e, f, grid, block, params, stream, true);
+        enqueue_persistent(e, precision, variant, d_tiles, n_tiles, total, d_counts, d_smooth,
+                           d_iters, d_queue, tn, stream, false, true);
+    });
+}
+
+int cu_escape_tiles_persistent_capture(cu_engine e, int precision, int variant,
+                                       const cu_tile_desc *d_tiles, uint32_t n_tiles,
+                                       uint64_t total, uint32_t *d_counts, double *d_smooth,
+                                       unsigned long long *d_iters, unsigned long long *d_queue,
+                                       const cu_tune *tune, cu_graph *out) {
+    cu_graph g = nullptr;
+    int rc = guarded([&] {
+        if (!out) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        const cu_tune tn = persistent_args(e, variant, d_tiles, n_tiles, d_counts, d_queue, tune);
+        if (total == 0) fail(CU_ERR_ILLEGAL_ARGUMENT, "no pixels to capture");
+        (void)real_name(precision);
+        std::lock_guard<std::mutex> lock(e->mutex);
+        ensure_device();
+        g = new cu_graph_s();
+        check(hipEventCreate(&g->start), "hipEventCreate");
+        check(hipEventCreate(&g->stop), "hipEventCreate");
+        hipStream_t cs = nullptr;
+        check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        cu_stream_s wrap{cs};
+        try {
+            check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                enqueue_persistent(e, precision, variant, d_tiles, n_tiles, total, d_counts, d_smooth,
+                                   d_iters, d_queue, tn, &wrap, true, false);
+            } catch (...) {
+                hipGraph_t partial = nullptr;
+                (void)hipStreamEndCapture(cs, &partial);
+                if (partial) (void)hipGraphDestroy(partial);
+                throw;
+            }
+            check(hipStreamEndCapture(cs, &g->graph), "hipStreamEndCapture");
+            check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        } catch (...) {
+            (void)hipStreamDestroy(cs);
+            throw;
+        }
+        check(hipStreamDestroy(cs), "hipStreamDestroy");
+        *out = g;
+    });
+    if (rc != CU_OK && g) (void)cu_graph_destroy(g);
+    return rc;
+}
+
+int cu_graph_launch(cu_graph g, cu_stream stream) {
+    return guarded([&] {
+        if (!g || !g->exec) fail(CU_ERR_ILLEGAL_ARGUMENT, "graph is NULL");
+        ensure_device();
+        hipStream_t st = stream ? stream->stream : nullptr;
+        check(hipEventRecord(g->start, st), "hipEventRecord");
+        check(hipGraphLaunch(g->exec, st), "hipGraphLaunch");
+        check(hipEventRecord(g->stop, st), "hipEventRecord");
+        g->launched = true;
+    });
+}
+
+int cu_graph_last_ms(cu_graph g, float *ms) {
+    return guarded([&] {
+        if (!g || !ms) fail(CU_ERR_ILLEGAL_ARGUMENT, "NULL argument");
+        if (!g->launched) fail(CU_ERR_ILLEGAL_STATE, "graph has not been launched");
+        ensure_device();
+        check(hipEventSynchronize(g->stop), "hipEventSynchronize");
+        check(hipEventElapsedTime(ms, g->start, g->stop), "hipEventElapsedTime");
+    });
+}
+
+int cu_graph_destroy(cu_graph g) {
+    return guarded([&] {
+        if (!g) return;
+        ensure_device();
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        if (g->start) (void)hipEventDestroy(g->start);
+        if (g->stop) (void)hipEventDestroy(g->stop);
+        delete g;
     });
 }
 

@@ -31,3 +31,12 @@ struct cu_engine_s {
     cu_tune tune{16, 32, 256, 16};  // defaults of every launch (cu_engine_configure)
     std::mutex mutex;
 };
+
+// A captured launch sequence (cu_escape_tiles_persistent_capture) and the events
+// around its last replay.
+struct cu_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool launched = false;
+};

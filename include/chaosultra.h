@@ -318,6 +318,22 @@ int cu_escape_tiles_persistent(cu_engine e, int precision, int variant,
                                unsigned long long *d_iters, unsigned long long *d_queue,
                                const cu_tune *tune, cu_stream stream);
 
+/* The persistent launch (same arguments as cu_escape_tiles_persistent, its clears
+ * included, and d_iters' slots cleared as well) captured into a HIP graph, so a
+ * batch of tiles runs as one replayable launch; every node is a kernel.  The
+ * graph keeps the device pointers it was captured with. */
+typedef struct cu_graph_s *cu_graph;
+int cu_escape_tiles_persistent_capture(cu_engine e, int precision, int variant,
+                                       const cu_tile_desc *d_tiles, uint32_t n_tiles,
+                                       uint64_t total_pixels, uint32_t *d_counts, double *d_smooth,
+                                       unsigned long long *d_iters, unsigned long long *d_queue,
+                                       const cu_tune *tune, cu_graph *out);
+/* One replay of the graph on `stream` (HIP events around it). */
+int cu_graph_launch(cu_graph g, cu_stream stream);
+/* Milliseconds of the last replay (synchronises on it). */
+int cu_graph_last_ms(cu_graph g, float *ms);
+int cu_graph_destroy(cu_graph g);
+
 /* Palette compose of a smooth plane: idx = L - (round(v) mod L) - 1
  * (mandelbrot.cu:27-35), RGBA8 out.  The palette is staged in LDS. */
 int cu_compose_smooth(cu_engine e, const double *d_smooth, uint32_t smooth_pitch,

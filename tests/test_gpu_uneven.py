@@ -137,3 +137,32 @@ def test_autotuner_pick_renders_the_same_pixels(gpu, engine):
     got, _, _ = render_frame(cu, engine, tiles, at.frame_tune(cu, p))
     want, _, _ = render_frame(cu, engine, tiles, cu.FrameTune(16, 16, 5, 0))
     assert np.array_equal(got, want)
+
+
+def test_persistent_capture_replays(gpu, engine):
+    """The autotuned engine's persistent launch as one captured graph: replays with
+    eager work between them give the eager launch's pixels and iteration count."""
+    cu = gpu
+    at = importlib.import_module("chaos-ultra_amd.autotune")
+    tiles = light_subset(cu, 24)
+    batch = at.PersistentBatch(cu, tiles)
+    tn = cu.Tune(16, 16, 64, 8)
+    batch.render(engine, tn)
+    cu.synchronize()
+    want = (batch.counts.download(np.uint32), batch.smooth.download(np.float64), batch.iters.total())
+    g = batch.capture(engine, tn)
+    other = at.PersistentBatch(cu, light_subset(cu, 5))
+    try:
+        for k in range(4):
+            batch.counts.fill_u32(0)
+            other.render(engine, cu.Tune(8, 32, 256, 16))  # eager work between replays
+            cu.synchronize()
+            g.replay()
+            assert g.last_ms() > 0.0
+            got = (batch.counts.download(np.uint32), batch.smooth.download(np.float64),
+                   batch.iters.total())
+            assert np.array_equal(got[0], want[0])
+            assert np.array_equal(got[1], want[1])
+            assert got[2] == want[2]
+    finally:
+        g.close()
