@@ -190,6 +190,16 @@ def uneven_leg(cu, engine):
         m = float(np.median(ms[1:]))
         out["tuned_persistent"] = {"render_ms": round(m, 3), "giga_iter_s": round(it / m / 1e6, 1),
                                    "iterations": int(it)}
+        # the winner as one captured launch (clears and kernel in a HIP graph)
+        g = batch.capture(engine, tn)
+        gms = []
+        for _ in range(4):
+            g.replay()
+            gms.append(g.last_ms())
+        g.close()
+        gm = float(np.median(gms[1:]))
+        out["tuned_persistent"]["graph_replay_ms"] = round(gm, 3)
+        out["tuned_persistent"]["graph_replay_giga_iter_s"] = round(batch.iters.total() / gm / 1e6, 1)
     out["autotune"] = {"engine": res["engine"], "tune": res["tune"], "ms": round(res["ms"], 3),
                        "pipeline_best": res["pipeline"]["best"],
                        "persistent_best": res.get("persistent", {}).get("best"),
@@ -199,14 +209,30 @@ def uneven_leg(cu, engine):
     return out
 
 
+def smooth_error(got, want, same):
+    """Relative smooth-value error |got - want| / max(|want|, 1) over the pixels whose
+    counts agree (`same`): p50, p99 and max."""
+    rel = np.abs(got - want) / np.maximum(np.abs(want), 1.0)
+    r = rel[same]
+    if r.size == 0:
+        return None
+    return {"p50": float(np.percentile(r, 50)), "p99": float(np.percentile(r, 99)),
+            "max": float(r.max()), "pixels": int(r.size)}
+
+
 def deep_leg(cu, engine):
     """1920x1080 deep-zoom frames (maxIter 50000): FP64/FP32-delta perturbation and
-    direct FP64, rate and mismatch against the committed double-double subset."""
+    direct FP64; rate, mismatch and smooth-value error against direct FP64 on the
+    full frame (at 1e-6 and 1e-10, where direct FP64 resolves the pixels) and
+    against the committed double-double subset (all widths)."""
     g = np.load(os.path.join(ROOT, "tests", "golden", "deep_dd_subset.npz"))
     idx = g["index"]
     out = {}
     for w in (1e-6, 1e-10, 1e-20):
         row = {}
+        planes = {}
+        dd_c = g[f"counts_{w:.0e}"].astype(np.uint32)
+        dd_s = g[f"smooth_{w:.0e}"].astype(np.float64)
         for name, method in (("perturb_f64", cu.DEEP_PERTURB_F64), ("perturb_f32", cu.DEEP_PERTURB_F32),
                              ("direct_f64", cu.DEEP_DIRECT_F64)):
             if method == cu.DEEP_DIRECT_F64 and w < 1e-13:
@@ -218,7 +244,9 @@ def deep_leg(cu, engine):
                 f.render()
                 ms.append(f.report().total_ms)
             rep = f.report()
-            counts = f.counts().ravel()[idx]
+            full_c, full_s = f.counts().ravel(), f.smooth().ravel()
+            planes[name] = (full_c, full_s)
+            counts, smooth = full_c[idx], full_s[idx]
             f.close()
             m = float(np.median(ms))
             flops = PERTURB_FLOPS_PER_ITER if method != cu.DEEP_DIRECT_F64 else FLOPS_PER_ITER
@@ -226,7 +254,8 @@ def deep_leg(cu, engine):
             tflops = rep.iterations * flops / (m * 1e-3) / 1e12
             row[name] = {"frame_ms": round(m, 3), "giga_iter_s": round(rep.iterations / m / 1e6, 1),
                          "tflops": round(tflops, 2), "frac_of_vector_peak": round(tflops / peak, 4),
-                         "mismatch_vs_dd_pct": round(100 * float(np.mean(counts != g[f"counts_{w:.0e}"])), 3)}
+                         "mismatch_vs_dd_pct": round(100 * float(np.mean(counts != dd_c)), 3),
+                         "smooth_rel_err_vs_dd": smooth_error(smooth, dd_s, counts == dd_c)}
             if method != cu.DEEP_DIRECT_F64:
                 row[name]["orbit_ms"] = round(rep.orbit_ms, 3)
                 # a zoom sequence toward the same centre keeps the orbit between frames
@@ -241,6 +270,11 @@ def deep_leg(cu, engine):
         if "direct_f64" in row:
             row["perturb_f32_over_direct_f64"] = round(row["perturb_f32"]["giga_iter_s"] /
                                                        row["direct_f64"]["giga_iter_s"], 3)
+            dc, ds = planes["direct_f64"]
+            for name in ("perturb_f64", "perturb_f32"):
+                c, sm = planes[name]
+                row[name]["mismatch_vs_direct_f64_pct"] = round(100 * float(np.mean(c != dc)), 3)
+                row[name]["smooth_rel_err_vs_direct_f64"] = smooth_error(sm, ds, c == dc)
         out[f"width_{w:.0e}"] = row
     return out
 
@@ -249,8 +283,10 @@ def renderer_leg(cu, w=3840, h=2160, frames=5):
     """The drop-in path (FractalRenderer): renderQuality on the module's default
     view (supplyDefaultValues: centre (-0.5, 0), zoom 2, maxIterations 1600,
     maxSuperSampling 5, adaptive) at 3840x2160, then renderFast frames zooming in
-    by 0.977 about the centre with reuse and foveation.  Iterations are the
-    module's executed-iteration count (cu_renderer_iterations)."""
+    by 0.977 about the centre with reuse and foveation, then renderQuality at the
+    frame pipeline's view (FP64 by the precision switch, maxIter 5000) with 1, 4
+    and 16 (adaptive) samples.  Iterations are the module's executed-iteration
+    count (cu_renderer_iterations)."""
     prov = cu.FractalRendererProvider()
     r = prov.get_default_renderer()
     try:
@@ -287,6 +323,31 @@ def renderer_leg(cu, w=3840, h=2160, frames=5):
             its.append(r.iterations() - i0)
         out["render_fast"] = {"frames": len(ms), "ms_p50": round(float(np.median(ms)), 3),
                               "giga_iter_s": round(float(np.sum(its)) / float(np.sum(ms)) / 1e6, 1)}
+        # the frame pipeline's view (C2 centre, width 1e-4) at 4K through renderQuality: the
+        # precision switch picks the FP64 kernel (pixel 2.6e-8 below the float ulp of 0.74)
+        seg = cu.plane_segment_from_center(CENTER[0], CENTER[1], 1e-4 * h / w, w, h)
+        out["c2_view_fp64"] = {}
+        for max_ss, adaptive in ((1.0, 0), (4.0, 0), (16.0, 1)):
+            m = cu.RenderingModel()
+            r.supply_default_values(m, w, h)
+            m.set_segment(*seg)
+            m.max_iterations = MAX_ITER
+            m.max_super_sampling = max_ss
+            m.use_adaptive_super_sampling = adaptive
+            m.use_sample_reuse = 0
+            m.use_foveated_rendering = 0
+            r.render_quality(m)  # warm-up
+            ms, its = [], []
+            for _ in range(frames):
+                i0, t0 = r.iterations(), time.perf_counter()
+                r.render_quality(m)
+                ms.append((time.perf_counter() - t0) * 1e3)
+                its.append(r.iterations() - i0)
+            k = int(np.argsort(ms)[len(ms) // 2])
+            out["c2_view_fp64"][f"max_ss_{max_ss:g}{'_adaptive' if adaptive else ''}"] = {
+                "ms_p50": round(ms[k], 3), "iterations": int(its[k]),
+                "giga_iter_s": round(its[k] / ms[k] / 1e6, 1),
+                "precision": "fp64" if m.floating_point_precision == cu.PRECISION_DOUBLE else "fp32"}
         r.free_rendering_resources()
         return out
     finally:
@@ -308,6 +369,48 @@ def serving_leg(cu, hbm_budget=16 << 30, slots=8):
                 "start_widths": "log-uniform in [1e-14, 3]; frames below 1e-12 use perturbation",
                 "resolutions": [f"{s['width']}x{s['height']}" for s in sessions]})
     return out
+
+
+def serving_capacity_leg(cu, frames=45, p99_target_ms=33.0, slots=8, hbm_budget=192 << 30,
+                         seed=20261203):
+    """How many 3840x2160 zoom sessions one GPU serves at 30 frames/s with the p99
+    arrival -> completion latency within p99_target_ms: sessions drawn as in the
+    serving leg but all at 4K, replayed in real time; the session count doubles from
+    8 until the p99 exceeds the target, then bisects between the last two counts."""
+    sv = importlib.import_module("chaos-ultra_amd.serve")
+
+    def run_n(n):
+        sessions = sv.zoom_sessions(n, frames, seed=seed, resolutions=((3840, 2160),))
+        server = sv.Server(cu.KERNELS_DIR, hbm_budget, slots=slots)
+        try:
+            recs, span = sv.run(server, sessions, realtime=True, timeout_s=120.0)
+            s = sv.summarise(recs, sessions, server.report(), span)
+        finally:
+            server.close()
+        log(f"[bench] serving capacity: {n} 4K sessions -> p50 {s['p50_ms']} ms, p99 {s['p99_ms']} ms, "
+            f"{s['giga_iter_s']} G-iter/s")
+        return {k: s[k] for k in ("p50_ms", "p99_ms", "max_ms", "giga_iter_s", "frames",
+                                  "frames_reuse", "frames_full", "frames_deep", "peak_bytes")}
+
+    runs = {}
+    ok, n = 0, 8
+    while n <= 512:
+        runs[n] = run_n(n)
+        if runs[n]["p99_ms"] > p99_target_ms:
+            break
+        ok, n = n, n * 2
+    lo, hi = ok, n
+    while hi - lo > max(1, lo // 8):
+        mid = (lo + hi) // 2
+        runs[mid] = run_n(mid)
+        if runs[mid]["p99_ms"] <= p99_target_ms:
+            lo = mid
+        else:
+            hi = mid
+    return {"sustainable_4k_sessions_at_30fps": lo, "p99_target_ms": p99_target_ms,
+            "frames_per_session": frames, "slots": slots, "hbm_budget_bytes": hbm_budget,
+            "shallow_precision": "fp64", "deep_method": "fp32 perturbation below width 1e-12",
+            "runs": {str(k): runs[k] for k in sorted(runs)}}
 
 
 def read_traffic():
@@ -434,6 +537,19 @@ def c1_leg(cu, engine, cpu: bool):
     return out
 
 
+def calibrate_copy(cu, mib):
+    """A device-to-device copy of `mib` MiB: a plain copy of known size that scales the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    if not mib:
+        return
+    n = mib << 20
+    src, dst = cu.DeviceBuffer(n), cu.DeviceBuffer(n)
+    src.fill_u32(1)
+    cu.abi.call("cu_memcpy_dtod", dst.ptr, src.ptr, n)
+    cu.synchronize()
+    log(f"[bench] copied {mib} MiB device-to-device for counter calibration")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -442,7 +558,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the grid / FP32 variants and the uneven-tile and deep-zoom legs")
-    ap.add_argument("--leg", choices=["all", "uneven", "deep", "serving", "renderer"], default="all",
+    ap.add_argument("--leg", choices=["all", "uneven", "deep", "serving", "serving_capacity", "renderer"],
+                    default="all",
                     help="run one leg only and print its JSON (used for profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device once (a plain copy "
@@ -458,7 +575,9 @@ def main():
     engine = cu.Engine()
     if args.leg != "all":
         leg = {"uneven": lambda: uneven_leg(cu, engine), "deep": lambda: deep_leg(cu, engine),
-               "serving": lambda: serving_leg(cu), "renderer": lambda: renderer_leg(cu)}[args.leg]()
+               "serving": lambda: serving_leg(cu), "renderer": lambda: renderer_leg(cu),
+               "serving_capacity": lambda: serving_capacity_leg(cu)}[args.leg]()
+        calibrate_copy(cu, args.calibrate_copy_mib)
         engine.close()
         print(json.dumps({"leg": args.leg, "result": leg}), flush=True)
         return
@@ -544,6 +663,8 @@ def main():
         log("[bench] serving: 16 zoom sessions in real time")
         variants["serving_16_sessions"] = serving_leg(cu)
         log(f"[bench] serving: {variants['serving_16_sessions']}")
+        log("[bench] serving capacity: 4K sessions at 30 fps within a 33 ms p99")
+        variants["serving_capacity_4k"] = serving_capacity_leg(cu)
         variants["renderer_4k"] = renderer_leg(cu)
         log(f"[bench] renderer (drop-in path) 3840x2160: {variants['renderer_4k']}")
         log("[bench] deep zoom 1920x1080")
@@ -590,13 +711,7 @@ def main():
         result["cpu_baseline"] = cpu_baseline(cu, tile, frame_counts, frame_smooth)
         log(f"[bench] CPU {result['cpu_baseline']['value']} G-iter/s, band vs GPU frame "
             f"{result['cpu_baseline'].get('vs_gpu_frame')}")
-    if args.calibrate_copy_mib:
-        n = args.calibrate_copy_mib << 20
-        src, dst = cu.DeviceBuffer(n), cu.DeviceBuffer(n)
-        src.fill_u32(1)
-        cu.abi.call("cu_memcpy_dtod", dst.ptr, src.ptr, n)
-        cu.synchronize()
-        log(f"[bench] copied {args.calibrate_copy_mib} MiB device-to-device for counter calibration")
+    calibrate_copy(cu, args.calibrate_copy_mib)
     engine.close()
     print(json.dumps(result), flush=True)
 
