@@ -19,7 +19,8 @@ algorithmic FLOP rate against the FP32 / FP64 vector peak: 15 FLOPs per
 perturbation step, 8 per direct step, whole frame including the reference orbit), and
 the roofline of the dominant kernel (pass 1 of the pipeline:
 8 algorithmic FP64 FLOPs per iteration it executes, against the 78.6 TFLOP/s
-FP64 vector peak and the 39.3 TFLOP/s add/mul-only ceiling), the thread-per-
+FP64 vector peak, and its iteration rate against the issue ceiling of the fused
+step's 6 FP64 VALU instructions: 39.3e12 lane-instructions/s / 6), the thread-per-
 sample grid kernel and the FP32 pipeline on the same frame, and the oracle's CPU
 rate on a bounded row band of the same frame on the host's cores.
 
@@ -54,6 +55,9 @@ FLOPS_PER_ITER = 8
 # fma(Zx, dx, Zy * dy) < -|Z|^2/2 (a product and an FMA, 3 FLOPs)
 PERTURB_FLOPS_PER_ITER = 15
 FP64_PEAK_TFLOPS = 78.6
+# the fused FP64 step issues 6 VALU instructions (3 mul, 2 add, 1 fma); FP64
+# instructions issue at 16 lanes per cycle per SIMD = FP64_PEAK_TFLOPS / 2 lane-ops/s
+FP64_ISSUE_CEILING_GITER = FP64_PEAK_TFLOPS / 2 * 1e3 / 6
 FP32_PEAK_TFLOPS = 157.3
 TUNE = (16, 24, 3)  # block_steps, handoff_lanes, tail_passes
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_fp64_pass1_traffic.md")
@@ -702,7 +706,9 @@ def main():
                      "kernel_ms": round(pass1_ms, 4),
                      "iterations_per_launch": int(iters1),
                      "flops_per_iteration": FLOPS_PER_ITER,
-                     "frac_of_add_mul_ceiling": round(achieved / (FP64_PEAK_TFLOPS / 2), 4),
+                     "issue_ceiling_giga_iter_s": round(FP64_ISSUE_CEILING_GITER, 1),
+                     "frac_of_issue_ceiling": round(iters1 / (pass1_ms * 1e-3) / 1e9
+                                                    / FP64_ISSUE_CEILING_GITER, 4),
                      "whole_frame_tflops": round(frame_tflops, 2)},
         "variants": variants,
     }

@@ -13,7 +13,8 @@ Candidates:
     length, resident waves per CU, pixels taken per queue grab and the number
     of idle lanes that trigger a refill.
 
-The search is coordinate descent from a default point, one knob at a time in
+Both engines are searched in the precision asked for (FP64 or FP32).  The
+search is coordinate descent from a default point, one knob at a time in
 the order above, each candidate timed with HIP events on its stream (median of
 `repeats` after `warmup` renders).  Every candidate renders the same pixels;
 the caller can check their outputs against each other.
@@ -95,8 +96,9 @@ def search_pipeline(measure, log=None):
 class PersistentBatch:
     """Device copy of a tile batch laid out back to back for the persistent kernel."""
 
-    def __init__(self, cu, tiles):
+    def __init__(self, cu, tiles, precision=64):
         self.cu = cu
+        self.precision = precision
         arr = (cu.TileDesc * len(tiles))()
         offset = 0
         for k, t in enumerate(tiles):
@@ -112,18 +114,18 @@ class PersistentBatch:
         self.smooth = cu.DeviceBuffer(offset * 8)
         self.iters = cu.IterationCounter()
         self.queue = cu.DeviceBuffer(8)
-        self.variant = 1 if all(cu.tile_fused_safe(t, 64) for t in tiles) else 0
+        self.variant = 1 if all(cu.tile_fused_safe(t, precision) for t in tiles) else 0
 
-    def render(self, engine, tune, precision=64) -> float:
+    def render(self, engine, tune) -> float:
         self.iters.clear()
-        engine.escape_tiles_persistent(precision, self.variant, self.d_tiles, self.n, self.total,
+        engine.escape_tiles_persistent(self.precision, self.variant, self.d_tiles, self.n, self.total,
                                        self.counts, self.smooth, self.iters, self.queue, tune)
         return engine.last_kernel_ms()
 
-    def capture(self, engine, tune, precision=64):
+    def capture(self, engine, tune):
         """The persistent launch (clears included) as one replayable HIP graph."""
         g = C.c_void_p()
-        self.cu.abi.call("cu_escape_tiles_persistent_capture", engine.handle, precision, self.variant,
+        self.cu.abi.call("cu_escape_tiles_persistent_capture", engine.handle, self.precision, self.variant,
                          self.d_tiles.ptr, self.n, C.c_uint64(self.total), self.counts.ptr,
                          self.smooth.ptr, self.iters.ptr, self.queue.ptr, C.byref(tune), C.byref(g))
         return CapturedLaunch(self.cu, g)
@@ -165,12 +167,12 @@ def tune_batch(cu, engine, tiles, precision=64, repeats=3, warmup=1, persistent=
     result = {"pipeline": {"best": pbest, "ms": pms, "table": ptable}}
     frame.close()
     best = ("pipeline", pbest, pms)
-    if persistent and precision == 64:
-        batch = PersistentBatch(cu, tiles)
+    if persistent:
+        batch = PersistentBatch(cu, tiles, precision)
 
         def measure_persistent(p):
             tn = cu.Tune(p["block_steps"], p["waves_per_cu"], p["chunk"], p["refill_lanes"])
-            return _median_ms(lambda: batch.render(engine, tn, precision), repeats, warmup)
+            return _median_ms(lambda: batch.render(engine, tn), repeats, warmup)
 
         qbest, qms, qtable = _descend(PERSISTENT_SPACE, PERSISTENT_START, measure_persistent, log)
         result["persistent"] = {"best": qbest, "ms": qms, "table": qtable}

@@ -46,8 +46,8 @@ def with_offsets(tiles):
     return offset
 
 
-def render_frame(cu, engine, tiles, tune):
-    f = cu.Frame(engine, tiles, 64, -1)
+def render_frame(cu, engine, tiles, tune, precision=64):
+    f = cu.Frame(engine, tiles, precision, -1)
     try:
         f.render(tune)
         cu.synchronize()
@@ -137,6 +137,23 @@ def test_autotuner_pick_renders_the_same_pixels(gpu, engine):
     got, _, _ = render_frame(cu, engine, tiles, at.frame_tune(cu, p))
     want, _, _ = render_frame(cu, engine, tiles, cu.FrameTune(16, 16, 5, 0))
     assert np.array_equal(got, want)
+
+
+def test_autotuner_fp32_searches_both_engines(gpu, engine):
+    """The FP32 search times the persistent kernel too, and whichever engine wins
+    renders the batch's pixels exactly as the FP32 pipeline does."""
+    cu = gpu
+    tiles = light_subset(cu, 16)
+    res = at.tune_batch(cu, engine, tiles, precision=32, repeats=1, warmup=0)
+    assert "persistent" in res and len(res["persistent"]["table"]) >= 6
+    want, want_s, want_it = render_frame(cu, engine, tiles, cu.FrameTune(16, 16, 5, 0), 32)
+    q = res["persistent"]["best"]
+    batch = at.PersistentBatch(cu, tiles, 32)
+    batch.render(engine, cu.Tune(q["block_steps"], q["waves_per_cu"], q["chunk"],
+                                 q["refill_lanes"]))
+    cu.synchronize()
+    assert np.array_equal(batch.counts.download(np.uint32), want)
+    assert batch.iters.total() == want_it
 
 
 def test_persistent_capture_replays(gpu, engine):
