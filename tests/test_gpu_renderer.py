@@ -63,6 +63,37 @@ def test_render_quality_mandelbrot(gpu, provider, oracle, w, h, max_ss, adaptive
         r.free_rendering_resources()
 
 
+@pytest.mark.parametrize("w,h,max_ss,adaptive", [(3840, 2160, 4.0, False),
+                                                 (1920, 1080, 16.0, True),
+                                                 (1920, 1080, 1.0, False)])
+def test_render_quality_c2_view_full_size(gpu, provider, oracle, w, h, max_ss, adaptive):
+    """The module kernels at full size on the 4096^2 frame's view (centre
+    (-0.743643887037151, 0.131825904205330), width 1e-4, maxIter 5000, FP64 by the
+    precision switch): every pixel's fields, the composed image and the executed
+    iterations equal the oracle's."""
+    cu = gpu
+    r = provider.get_default_renderer()
+    pal = cu.default_palette()
+    r.initialize_rendering(w, h, pal)
+    try:
+        seg = cu.plane_segment_from_center(-0.743643887037151, 0.131825904205330, 1e-4 * h / w,
+                                           w, h)
+        m = model_for(cu, seg, 5000, max_ss, adaptive)
+        before = r.iterations()
+        r.render_quality(m)
+        assert m.error_count == 0, m.last_error
+        assert m.floating_point_precision == cu.PRECISION_DOUBLE
+        got = r.read_pixel_info(primary=True)
+        flags = cu.FLAG_ADAPTIVE_SS if adaptive else 0
+        oracle.render_iterations()
+        want = oracle.render_main(64, cu.FRACTAL_MANDELBROT, (0, 0), w, h, seg, 5000, max_ss, flags)
+        assert r.iterations() - before == oracle.render_iterations()
+        fields_equal(got, want)
+        assert np.array_equal(r.read_output(), oracle.compose(want, pal, max_ss, False))
+    finally:
+        r.free_rendering_resources()
+
+
 @pytest.mark.parametrize("zoom,prec", [(3.0, 32), (1e-6, 64), (1e-15, 64)])
 def test_render_quality_precision_switch(gpu, provider, oracle, zoom, prec):
     cu = gpu
