@@ -114,14 +114,17 @@ __device__ __forceinline__ void count_iterations(uint64_t v) {
         atomicAdd(cu_iteration_slots + (wave & (CU_ITER_SLOTS - 1u)), (unsigned long long)v);
 }
 
-// computeDispersion (fractalRendererGeneric.cu:24-34) over the stored samples;
-// the array lives in registers, so it is indexed with unrolled selects.
-__device__ __forceinline__ float dispersion(const float (&samples)[kAdaptiveThreshold], uint n,
-                                            float mean) {
+// computeDispersion (fractalRendererGeneric.cu:24-34) over the stored samples of
+// one lane's pixel (the first kAdaptiveThreshold; the reference's array holds no
+// more), kept in the wave's LDS as [round][lane] so that they take no registers.
+__device__ __forceinline__ float dispersion(const float (&samples)[kAdaptiveThreshold][64],
+                                            uint lane, uint n, float mean) {
     float variance = 0;
-#pragma unroll
-    for (uint j = 0; j < kAdaptiveThreshold; ++j)
-        if (j < n) variance += (samples[j] - mean) * (samples[j] - mean);
+    const uint stored = n < kAdaptiveThreshold ? n : kAdaptiveThreshold;
+    for (uint j = 0; j < stored; ++j) {
+        const float v = samples[j][lane];
+        variance += (v - mean) * (v - mean);
+    }
     variance /= (n - 1);
     return variance / mean;
 }
@@ -136,16 +139,26 @@ __device__ __forceinline__ Point<uint> image_indexes_at(uint t) {
 }
 
 // Sample offsets of sample i of a pixel asked for sampleCountF samples (:105-117).
+// The first three lie on the diagonal at i / 3, the three quotients rounded once
+// at compile time; the rest use the pixel's row constants (sample_rows).
+constexpr uint kSkipFirst = 2;
+
+__device__ __forceinline__ void sample_rows(float sample_count_f, float &per_row_f,
+                                           uint &per_row_i) {
+    per_row_f = sample_count_f > (float)kSkipFirst ? sqrtf(sample_count_f - kSkipFirst) : 1.f;
+    per_row_i = (uint)roundf(per_row_f);
+}
+
 template <class Real>
-__device__ __forceinline__ void sample_offset(uint i, float sample_count_f, Real &dx, Real &dy) {
-    const uint skip_first = 2;
-    if (i <= skip_first) {
-        dx = dy = (Real)i / (Real)(skip_first + 1);
+__device__ __forceinline__ void sample_offset(uint i, float per_row_f, uint per_row_i, Real &dx,
+                                              Real &dy) {
+    if (i <= kSkipFirst) {
+        constexpr Real third = (Real)1 / (Real)(kSkipFirst + 1);
+        constexpr Real two_thirds = (Real)2 / (Real)(kSkipFirst + 1);
+        dx = dy = i == 0 ? (Real)0 : (i == 1 ? third : two_thirds);
     } else {
-        const float per_row_f = sqrtf(sample_count_f - skip_first);
-        const uint per_row_i = (uint)roundf(per_row_f);
-        dx = (float)((i - skip_first) % per_row_i) / per_row_f;
-        dy = (float)((i - skip_first) / per_row_i) / per_row_f;
+        dx = (float)((i - kSkipFirst) % per_row_i) / per_row_f;
+        dy = (float)((i - kSkipFirst) / per_row_i) / per_row_f;
     }
 }
 
@@ -171,8 +184,9 @@ constexpr unsigned long long kAbsent = ~0ull;  // above any iteration count
 // One wave's LDS: each pixel's results of the rounds not yet voted on
 // (iterations executed, kAbsent until posted) and the hand-out slots.
 struct WaveLds {
-    unsigned long long result[kRing][64];  // [sample % kRing][owner lane]
-    uint handout[64];        // owner lane | sample << 8, by rank
+    unsigned long long result[kRing][64];   // [sample % kRing][owner lane]
+    float samples[kAdaptiveThreshold][64];  // the owner's first sample values, [round][lane]
+    uint handout[64];                       // owner lane | sample << 8, by rank
 };
 
 // Orders this wave's LDS accesses across the phases below (the LDS serves a
@@ -216,9 +230,6 @@ __device__ uint sample_the_fractal(WaveLds &lds, int site, Point<uint> grid, Rec
     bool in_loop = site != 0;
     uint sample_count = 0, escape_sum = 0, result = 0;
     uint confirmed = 0, issued = 0;  // rounds voted on; samples handed out
-    float samples[kAdaptiveThreshold];
-#pragma unroll
-    for (uint j = 0; j < kAdaptiveThreshold; ++j) samples[j] = 0;
     if (in_loop) {
         if (sample_count_f < 1) {  // :87-90
             sample_count_f = 0;
@@ -227,7 +238,9 @@ __device__ uint sample_the_fractal(WaveLds &lds, int site, Point<uint> grid, Rec
             sample_count = min(kMaxSuperSampling, (uint)roundf(sample_count_f));
         }
     }
-    const float count_f = sample_count_f;  // the offsets use the caller's sampleCountF
+    float per_row_f;  // the offsets use the caller's sampleCountF
+    uint per_row_i;
+    sample_rows(sample_count_f, per_row_f, per_row_i);
 #pragma unroll
     for (uint r = 0; r < kRing; ++r) lds.result[r][lane] = kAbsent;
     // pixelSize = image.size() / gridSize.cast<Real>()  (:97)
@@ -258,19 +271,19 @@ __device__ uint sample_the_fractal(WaveLds &lds, int site, Point<uint> grid, Rec
                 const uint escape_time = (uint)Fractal::value(n_here, max_iter);
                 executed += n_here;
                 escape_sum += escape_time;
-#pragma unroll
-                for (uint j = 0; j < kAdaptiveThreshold; ++j)
-                    if (j == ri) samples[j] = (float)escape_time;
+                if (ri < kAdaptiveThreshold) lds.samples[ri][lane] = (float)escape_time;
             }
             const bool part = ready && vote_round(ri, sample_count, adaptive);
             float disp = 0;
             if (part) {
                 const float mean = escape_sum / (ri + 1);  // integer mean (:135)
-                disp = dispersion(samples, ri, mean);
+                disp = dispersion(lds.samples, lane, ri, mean);
             }
             const uint64_t P = cu::ballot(part) & group;
             const bool v1 =
-                (cu::ballot(part && fabsf(samples[0] - samples[1]) < FLT_EPSILON) & group) == P;
+                (cu::ballot(part && fabsf(lds.samples[0][lane] - lds.samples[1][lane]) <
+                                        FLT_EPSILON) &
+                 group) == P;
             const bool v2 = (cu::ballot(part && disp < 0.01f) & group) == P;
             const bool v3 = (cu::ballot(part && disp <= 1.f) & group) == P;
             if (part) {
@@ -325,13 +338,14 @@ __device__ uint sample_the_fractal(WaveLds &lds, int site, Point<uint> grid, Rec
                 if (take) h = lds.handout[__popcll(idle & lanes_below)];
                 wave_lds_fence();
                 const uint src = h & 63u;
-                const float ocount = __shfl(count_f, (int)src, 64);
+                const float orow_f = __shfl(per_row_f, (int)src, 64);
+                const uint orow_i = (uint)__shfl((int)per_row_i, (int)src, 64);
                 if (take) {
                     owner = src;
                     sample = h >> 8;
                     const Point<uint> op = image_indexes_at(wave_base + owner);
                     Real dx, dy;
-                    sample_offset<Real>(sample, ocount, dx, dy);
+                    sample_offset<Real>(sample, orow_f, orow_i, dx, dy);
                     // c = leftTop + (1,-1) * (pixel + delta) * pixelSize  (:119-124)
                     const Real X = image.left_bottom.x + ((Real)op.x + dx) * psx;
                     const Real Y = image.right_top.y - ((Real)op.y + dy) * psy;
@@ -344,17 +358,20 @@ __device__ uint sample_the_fractal(WaveLds &lds, int site, Point<uint> grid, Rec
         }
 
         // ---- unchecked blocks until kRefill lanes wait (idle ones count while work is left) ----
+        Real x2 = x * x, y2 = y * y;  // squares of the current state, carried
         for (;;) {
             if (mode == 1) {
                 sx = x;
                 sy = y;
 #pragma unroll
-                for (int k = 0; k < K / 2; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+                for (int k = 0; k < K / 2; ++k)
+                    cu::quad_step_sq<Real, kFused>(x, y, x2, y2, cx, cy);
                 mx = x;
                 my = y;
 #pragma unroll
-                for (int k = K / 2; k < K; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
-                if (x * x + y * y < bsq) {
+                for (int k = K / 2; k < K; ++k)
+                    cu::quad_step_sq<Real, kFused>(x, y, x2, y2, cx, cy);
+                if (x2 + y2 < bsq) {
                     i += K;
                     if (max_iter - i < (uint)K) mode = 2;
                 } else {
