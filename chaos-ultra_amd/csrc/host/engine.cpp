@@ -330,6 +330,7 @@ int cu_graph_destroy(cu_graph g) {
     return guarded([&] {
         if (!g) return;
         ensure_device();
+        if (g->launched) (void)hipEventSynchronize(g->stop);  // the last replay has ended
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (g->graph) (void)hipGraphDestroy(g->graph);
         if (g->start) (void)hipEventDestroy(g->start);

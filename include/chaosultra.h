@@ -332,6 +332,7 @@ int cu_escape_tiles_persistent_capture(cu_engine e, int precision, int variant,
 int cu_graph_launch(cu_graph g, cu_stream stream);
 /* Milliseconds of the last replay (synchronises on it). */
 int cu_graph_last_ms(cu_graph g, float *ms);
+/* Waits for the last replay, then frees the graph. */
 int cu_graph_destroy(cu_graph g);
 
 /* Palette compose of a smooth plane: idx = L - (round(v) mod L) - 1
