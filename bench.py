@@ -10,7 +10,8 @@ value = Giga-iterations/s over the timed steps (iterations executed; an
 interior sample counts maxIter).
 
 Also reported (variants): the reference's CPU-runnable 1024x1024 frame (maxIter
-1000) on the GPU and in full through the oracle on the host, the 256-tile uneven batch (autotuned, captured
+1000) on the GPU (the CPU-baseline leg runs it in full through the oracle on the
+host and compares the counts), the 256-tile uneven batch (autotuned, captured
 replay), 16 zoom sessions through the continuous-batching loop (p50/p99
 arrival-to-completion latency), the drop-in FractalRenderer path at 3840x2160
 (renderQuality and renderFast), the 1920x1080 deep-zoom frames (FP32 and FP64-delta perturbation and
@@ -427,11 +428,13 @@ def read_traffic():
         return None
 
 
-def cpu_baseline(cu, tile, gpu_counts=None, gpu_smooth=None, seconds_target=15.0):
+def cpu_baseline(cu, tile, gpu_counts=None, gpu_smooth=None, c1=None, seconds_target=15.0):
     """The oracle (a scalar C++ restatement of the reference iteration, OpenMP over rows)
     on a bounded row band of the same frame.  The band's counts and smooth values are
     compared with the same rows of the GPU's frame (gpu_counts / gpu_smooth, the
-    full 4096x4096 planes of the timed render)."""
+    full 4096x4096 planes of the timed render).  With c1 = (tile, pipeline counts,
+    grid-kernel counts) of the 1024x1024 frame, that frame also runs in full through
+    the oracle and its counts are compared with both GPU engines'."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle
     threads = os.cpu_count() or 1
@@ -470,23 +473,37 @@ def cpu_baseline(cu, tile, gpu_counts=None, gpu_smooth=None, seconds_target=15.0
                     break
     except OSError:
         pass
+    c1_check = {}
+    if c1 is not None:
+        c1_tile, c1_pipe, c1_grid = c1
+        c1_dts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            want, _, cit = _oracle.escape_tile(c1_tile, 64, threads=threads)
+            c1_dts.append(time.perf_counter() - t0)
+        c1_check = {"ms_p50": round(float(np.median(c1_dts)) * 1e3, 2),
+                    "giga_iter_s": round(cit / float(np.median(c1_dts)) / 1e9, 3),
+                    "threads": threads,
+                    "same_counts": bool(np.array_equal(c1_pipe, want) and
+                                        np.array_equal(c1_grid, want))}
     return {"value": round(float(np.median(rates)), 4), "unit": "Giga-iter/s", "cores": threads,
             "cpu_model": cpu_model, "kind": "port",
             "sample": f"rows {a}..{a + rows - 1} of the 4096x4096 FP64 frame (4 spp, maxIter 5000), "
                       f"{it} iterations per run, median of 3 runs ({', '.join(f'{d:.2f}' for d in dts)} s); "
                       f"g++ -O2 -ffp-contract=off, OpenMP {threads} threads",
-            **({"vs_gpu_frame": check} if check else {})}
+            **({"vs_gpu_frame": check} if check else {}),
+            **({"c1_frame_in_full": c1_check} if c1_check else {})}
 
 
-def c1_leg(cu, engine, cpu: bool):
+def c1_leg(cu, engine):
     """The reference's CPU-runnable case: the 1024x1024 FP64 frame, centre (-0.5, 0),
     width 3, 1 sample at each pixel centre, maxIter 1000, bailout 4 (counts and
     smooth values), through the frame pipeline and through the one-launch grid
-    kernel; and, unless cpu is False, the same frame in full through the oracle
-    on all host threads.  The headline figures are the faster engine's: 97 % of
+    kernel.  The headline figures are the faster engine's: 97 % of
     this frame's iterations belong to the 17 % of its pixels inside the set, which
     run all 1000 steps; the pipeline hands them on to its tail passes, while the
-    grid kernel runs them to the end in place."""
+    grid kernel runs them to the end in place.  Returns the JSON entry and the
+    tile with the pipeline's counts, for the CPU baseline leg's full-frame check."""
     tile = cu.make_tile(-2.0, -1.5, 1.0, 1.5, 1024, 1024, 1000, 4.0)
     fr = cu.Frame(engine, [tile], 64, -1)
     tn = cu.FrameTune(*TUNE, 0, 0)
@@ -524,21 +541,7 @@ def c1_leg(cu, engine, cpu: bool):
                         "tune": list(TUNE)},
            "grid_kernel": {"kernel_ms_p50": round(grid_ms, 4), "giga_iter_s": round(git / grid_ms / 1e6, 1),
                            "iterations": git, "same_counts_as_pipeline": bool(np.array_equal(gcounts, counts))}}
-    if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import _oracle
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        dts = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            want, _, cit = _oracle.escape_tile(tile, 64, threads=threads)
-            dts.append(time.perf_counter() - t0)
-        out["cpu_oracle"] = {"ms_p50": round(float(np.median(dts)) * 1e3, 2),
-                             "giga_iter_s": round(cit / float(np.median(dts)) / 1e9, 3),
-                             "threads": threads, "same_counts": bool(
-                                 np.array_equal(counts.reshape(1024, 1024), want) and
-                                 np.array_equal(gcounts.reshape(1024, 1024), want))}
-    return out
+    return out, (tile, counts.reshape(1024, 1024), gcounts.reshape(1024, 1024))
 
 
 def calibrate_copy(cu, mib):
@@ -640,9 +643,10 @@ def main():
     run.close()
 
     variants = {}
+    c1_planes = None
     if not args.no_variants:
-        log("[bench] C1 frame (1024x1024, maxIter 1000), GPU and the oracle on the host")
-        variants["c1_plain_1024"] = c1_leg(cu, engine, cpu=not args.no_cpu)
+        log("[bench] C1 frame (1024x1024, maxIter 1000)")
+        variants["c1_plain_1024"], c1_planes = c1_leg(cu, engine)
         log(f"[bench] C1: {variants['c1_plain_1024']}")
         vsteps = max(3, args.steps // 2)
         g = GridRun(cu, engine, tile, 64)
@@ -714,7 +718,7 @@ def main():
     }
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, bounded sample)")
-        result["cpu_baseline"] = cpu_baseline(cu, tile, frame_counts, frame_smooth)
+        result["cpu_baseline"] = cpu_baseline(cu, tile, frame_counts, frame_smooth, c1_planes)
         log(f"[bench] CPU {result['cpu_baseline']['value']} G-iter/s, band vs GPU frame "
             f"{result['cpu_baseline'].get('vs_gpu_frame')}")
     calibrate_copy(cu, args.calibrate_copy_mib)
