@@ -69,11 +69,11 @@ __global__ __launch_bounds__(64) void staticReferenceOrbit(double cx_hi, double 
 // The block shapes and LDS stages of perturbFloat / perturbDouble (perturb.hip).
 __global__ __launch_bounds__(256, 8) void staticPerturbFloat(DeepArgs a,
                                                              const OrbitEntry<float> *__restrict__ orbit) {
-    perturb_frame<float, 16, 2, 2, 20u * 1024u>(a, orbit);
+    perturb_frame<float, 32, 2, 2, 20u * 1024u>(a, orbit);
 }
 __global__ __launch_bounds__(512, 8) void staticPerturbDouble(DeepArgs a,
                                                               const OrbitEntry<double> *__restrict__ orbit) {
-    perturb_frame<double, 16, 4, 2, 40u * 1024u>(a, orbit);
+    perturb_frame<double, 32, 4, 2, 40u * 1024u>(a, orbit);
 }
 
 template <class Real>

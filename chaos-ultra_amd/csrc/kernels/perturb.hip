@@ -20,15 +20,16 @@ extern "C" __global__ __launch_bounds__(64) void referenceOrbitDouble(
 // `orbit` repeats a.orbit as its own read-only argument, so uniform loads from
 // it can go through the scalar cache.  The second bound (8 waves per SIMD)
 // keeps the kernels within 64 VGPRs, so that a CU's 32 wave slots can all be
-// filled; without it the compiler keeps a whole 16-step block's orbit reads in
-// flight.  Block shapes: perturb_frame.
+// filled; without it the compiler keeps a whole block's orbit reads in flight.
+// Blocks of 32 steps: against 16, 7-9 % faster at 1e-10 and 1e-20 and the same at
+// 1e-6, where a rebase reruns more of a longer block.  Block shapes: perturb_frame.
 extern "C" __global__ __launch_bounds__(256, 8) void perturbFloat(
     DeepArgs a, const OrbitEntry<float> *__restrict__ orbit) {
-    perturb_frame<float, 16, 2, 2, 20u * 1024u>(a, orbit);
+    perturb_frame<float, 32, 2, 2, 20u * 1024u>(a, orbit);
 }
 extern "C" __global__ __launch_bounds__(512, 8) void perturbDouble(
     DeepArgs a, const OrbitEntry<double> *__restrict__ orbit) {
-    perturb_frame<double, 16, 4, 2, 40u * 1024u>(a, orbit);
+    perturb_frame<double, 32, 4, 2, 40u * 1024u>(a, orbit);
 }
 extern "C" __global__ __launch_bounds__(256) void deepDirectDoubleExact(DeepArgs a) {
     deep_direct<16, false>(a);

@@ -27,7 +27,7 @@
 
 namespace {
 
-constexpr int K = 16;  // steps per unchecked block (perturbFloat / perturbDouble)
+constexpr int K = 32;  // steps per unchecked block (perturbFloat / perturbDouble)
 constexpr uint32_t kWidth = 1920, kHeight = 1080;
 
 inline float fma_r(float a, float b, float c) { return fmaf(a, b, c); }

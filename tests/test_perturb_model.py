@@ -1,6 +1,6 @@
 """The perturbation kernel's schedule, modelled on the CPU, against the oracle.
 
-tests/emulate/perturb_waves.cpp runs perturb_pixel's wave control flow (16-step
+tests/emulate/perturb_waves.cpp runs perturb_pixel's wave control flow (32-step
 unchecked blocks for the whole wave, a rerun up to the first step where a rebase
 test fires, escape tested once per block and replayed with checked steps) on
 8x8-pixel waves of 1920x1080 deep frames, and compares every pixel's count, state
