@@ -50,6 +50,28 @@ def test_c2_frame_pipeline_vs_oracle(gpu, engine, oracle, tune, precision):
     assert_smooth_close(smooth.reshape(384, 512), want_s)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_c2_full_frame_bands_vs_oracle(gpu, engine, oracle, precision):
+    """The 4096x4096 4-spp frame itself (SplitMix64 seed 20261202, the benchmark's
+    tile and tune) through the pipeline with compose; three row bands (top, centre,
+    bottom: 384 rows each) run through the oracle on the host's threads and must
+    match the GPU frame's counts, smooth values and palette colours."""
+    cu = gpu
+    seg = cu.plane_segment_from_center(*C2_CENTER, 1e-4, 4096, 4096)
+    tile = cu.make_tile(*seg, 4096, 4096, 5000, 256.0, spp=4, pattern=cu.SAMPLE_JITTER,
+                        seed=20261202)
+    pal = ((np.arange(256, dtype=np.uint32) * 0x00010305) | 0xFF000000).astype(np.uint32)
+    counts, smooth, it, rgba = render(cu, engine, [tile], precision, -1,
+                                      cu.FrameTune(16, 24, 3, 0), palette=pal)
+    counts, smooth, rgba = (a.reshape(4096, 4096) for a in (counts, smooth, rgba))
+    for r0 in (0, 1856, 3712):
+        want_c, want_s, _ = oracle.escape_tile(tile, precision, rows=(r0, r0 + 384))
+        assert np.array_equal(counts[r0:r0 + 384], want_c[r0:r0 + 384]), r0
+        assert_smooth_close(smooth[r0:r0 + 384], want_s[r0:r0 + 384])
+        want_rgba = oracle.compose_smooth(want_s[r0:r0 + 384], pal)
+        assert np.array_equal(rgba[r0:r0 + 384], want_rgba), r0
+
+
 def test_pass1_iterations_are_its_own(gpu, engine, oracle):
     """cu_frame_iterations' pass-1 figure counts the iterations pass 1 itself ran,
     including the steps of the samples it hands on, and the tail passes count only

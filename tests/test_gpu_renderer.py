@@ -137,6 +137,34 @@ def test_render_quality_julia_custom_params(gpu, provider, oracle):
         r.free_rendering_resources()
 
 
+def test_render_quality_julia_full_hd(gpu, provider, oracle):
+    """The Julia module at 1920x1080 on a view narrow enough for the FP64 kernel
+    (c = -0.7269 + 0.1889i, width 2e-5 about (0.4, 0.05), where counts run from
+    about 200 to 2000; maxIter 4000, 8 adaptive samples): every pixel, the composed
+    image and the executed iterations equal the oracle's."""
+    cu = gpu
+    r = provider.get_renderer("julia")
+    w, h = 1920, 1080
+    pal = cu.default_palette()
+    r.initialize_rendering(w, h, pal)
+    try:
+        r.set_fractal_custom_params("-0.7269;0.1889")
+        seg = cu.plane_segment_from_center(0.4, 0.05, 2e-5 * h / w, w, h)
+        m = model_for(cu, seg, 4000, 8.0, True)
+        before = r.iterations()
+        r.render_quality(m)
+        assert m.error_count == 0, m.last_error
+        assert m.floating_point_precision == cu.PRECISION_DOUBLE
+        oracle.render_iterations()
+        want = oracle.render_main(64, cu.FRACTAL_JULIA, (-0.7269, 0.1889), w, h, seg, 4000, 8.0,
+                                  cu.FLAG_ADAPTIVE_SS)
+        assert r.iterations() - before == oracle.render_iterations()
+        fields_equal(r.read_pixel_info(primary=True), want)
+        assert np.array_equal(r.read_output(), oracle.compose(want, pal, 8.0, False))
+    finally:
+        r.free_rendering_resources()
+
+
 def zoom_segment(seg, focus, w, h, coeff):
     """Zoom the segment about the pixel `focus` (RenderingController.zoomAt's geometry)."""
     lbx, lby, rtx, rty = seg
