@@ -16,7 +16,8 @@ Candidates:
 Both engines are searched in the precision asked for (FP64 or FP32).  The
 search is coordinate descent from a default point, one knob at a time in
 the order above, each candidate timed with HIP events on its stream (median of
-`repeats` after `warmup` renders).  Every candidate renders the same pixels;
+`repeats` after `warmup` renders) right after the incumbent is timed again, so
+that the two are compared at the same GPU clock.  Every candidate renders the same pixels;
 the caller can check their outputs against each other.
 """
 from __future__ import annotations
@@ -58,6 +59,20 @@ def _median_ms(fn, repeats, warmup):
     return float(np.median(ts))
 
 
+def _challenge(best, best_ms, cand, measure, table, log):
+    """Time `cand` right after timing the incumbent again, so that both run at the
+    same clock (the GPU's clock drifts over a search, and a process's first renders
+    run while it ramps up); returns the winner and its time."""
+    best_now = measure(best)
+    ms = measure(cand)
+    table.append((cand, ms))
+    if log:
+        log(f"[autotune] {cand} {ms:.3f} ms (incumbent {best_now:.3f} ms)")
+    if ms < best_now:
+        return cand, ms
+    return best, min(best_ms, best_now)
+
+
 def _descend(space, start, measure, log):
     best = dict(start)
     best_ms = measure(best)
@@ -68,12 +83,7 @@ def _descend(space, start, measure, log):
                 continue
             cand = dict(best)
             cand[knob] = v
-            ms = measure(cand)
-            table.append((cand, ms))
-            if log:
-                log(f"[autotune] {cand} {ms:.3f} ms")
-            if ms < best_ms:
-                best, best_ms = cand, ms
+            best, best_ms = _challenge(best, best_ms, cand, measure, table, log)
     return best, best_ms, table
 
 
@@ -84,12 +94,7 @@ def search_pipeline(measure, log=None):
     best, best_ms, table = _descend(PIPELINE_SPACE, PIPELINE_START, measure, log)
     for refill in DRAIN_ONLY_REFILL:
         cand = dict(best, tail_passes=0, drain_refill=refill)
-        ms = measure(cand)
-        table.append((cand, ms))
-        if log:
-            log(f"[autotune] {cand} {ms:.3f} ms")
-        if ms < best_ms:
-            best, best_ms = cand, ms
+        best, best_ms = _challenge(best, best_ms, cand, measure, table, log)
     return best, best_ms, table
 
 

@@ -48,3 +48,16 @@ def test_search_never_pairs_no_tail_pass_with_no_drain():
     at.search_pipeline(measure)
     assert all(p["tail_passes"] > 0 or p["drain_refill"] > 0 for p in seen)
     assert all(p["tail_passes"] <= 7 or p["drain_refill"] == 0 for p in seen)
+
+
+def test_slow_first_measurement_does_not_lose_the_start_point():
+    # the GPU's clock ramps up over a process's first renders: the start point's
+    # first timing is inflated, yet a worse candidate must not beat it
+    calls = []
+
+    def measure(p):
+        calls.append(dict(p))
+        true = 10.0 + sum(abs(p[k] - v) for k, v in at.PIPELINE_START.items())
+        return true * (1.5 if len(calls) == 1 else 1.0)
+    best, ms, _ = at.search_pipeline(measure)
+    assert best == at.PIPELINE_START and ms == 10.0
