@@ -343,6 +343,18 @@ __device__ __forceinline__ void add_iterations(unsigned long long *iters, uint64
     if (iters && __lane_id() == 0 && v)
         atomicAdd(iters + (wave & (CU_ITER_SLOTS - 1u)), (unsigned long long)v);
 }
+// Column of pixel blocks this workgroup covers in a 2-D grid over a frame.
+// Consecutive workgroups go round-robin to the 8 XCDs and then to their CUs, so
+// with a grid 64 blocks wide every CU's workgroups share blockIdx.x: a CU works
+// one column of the frame, and the CUs over the set's interior (orbits that run
+// all max_iter steps) end last while the rest idle.  Each row of blocks is
+// rotated by a golden-ratio fraction of the width, so a CU's workgroups land on
+// columns spread over the frame.  A rotation per row maps blocks one to one.
+__device__ __forceinline__ uint skewed_block_x() {
+    const uint gx = gridDim.x;
+    const uint skew = (uint)(((uint64_t)gx * 40503u) >> 16);  // 0.618 * gx
+    return (uint)(((uint64_t)blockIdx.x + (uint64_t)blockIdx.y * skew) % gx);
+}
 // This lane's bit of a wave-uniform lane mask (the inverse of ballot): the mask
 // becomes the execution mask itself, with no per-lane shift and compare.
 __device__ __forceinline__ bool in_mask(uint64_t mask) {

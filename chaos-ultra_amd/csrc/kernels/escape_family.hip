@@ -62,7 +62,7 @@ __device__ __forceinline__ bool tile_blocks_valid(const cu_tile_desc &t) {
 template <class Real, int K, bool kFused>
 __device__ void escape_tile(const cu_tile_desc &t, double smooth_c, uint *counts, double *smooth,
                             unsigned long long *iters) {
-    const uint px = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint px = cu::skewed_block_x() * blockDim.x + threadIdx.x;
     const uint py = blockIdx.y * blockDim.y + threadIdx.y;
     uint64_t executed = 0;
     if (px < t.width && py < t.height) {
@@ -113,7 +113,7 @@ __device__ void escape_tile_samples(const cu_tile_desc &t, double smooth_c, uint
     uint bw, bh;
     wave_pixel_block(spp, bw, bh);
     const uint pix = lane / spp, s = lane % spp;
-    const uint px = (blockIdx.x * 2u + (wave & 1u)) * bw + pix % bw;
+    const uint px = (cu::skewed_block_x() * 2u + (wave & 1u)) * bw + pix % bw;
     const uint py = (blockIdx.y * 2u + (wave >> 1)) * bh + pix / bw;
     const bool inside = px < t.width && py < t.height;
     const bool julia = t.kind == CU_FRACTAL_JULIA;
