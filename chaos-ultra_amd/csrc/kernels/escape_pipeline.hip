@@ -180,7 +180,7 @@ __device__ __forceinline__ void store_sample(const PipeArgs &a, uint tile, uint 
     a.sample_smooth[so] = smooth;
 }
 
-// Pass 1: wave w of the launch takes wave-block w of the frame.  The wave
+// Pass 1: wave w of the launch takes one wave-block of the frame.  The wave
 // number, tile and block geometry are wave-uniform (scalar registers); spp and
 // the block width are powers of two, so lane -> (pixel, sample) is shifts.
 template <class Real, int K, bool kFused>
@@ -202,7 +202,13 @@ __device__ void pass1(const PipeArgs a, uint total_blocks, Straggler<Real> *out_
     const uint lbw = (uint)__builtin_ctz(bw);
     const uint b = wave_global - a.block_offset[tile];
     const uint nbx = (t.width + bw - 1) >> lbw;
-    const uint bx = b % nbx, by = b / nbx;
+    // Row `by` of wave blocks is rotated by 0.618 of its width: workgroups go
+    // round-robin to the XCDs and their CUs, so without it a CU's workgroups
+    // would share a column of the frame (256 workgroups per row at 4096 px, 4 spp)
+    // and the CUs over the set's interior would run its long orbits.
+    const uint by = b / nbx;
+    const uint rot = (uint)(((uint64_t)nbx * 40503u) >> 16);
+    const uint bx = (uint)(((uint64_t)(b % nbx) + (uint64_t)by * rot) % nbx);
     const uint pix = lane >> lspp, s = lane & (spp - 1u);
     const uint px = bx * bw + (pix & (bw - 1u));
     const uint py = by * bh + (pix >> lbw);
