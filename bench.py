@@ -60,8 +60,16 @@ FP64_PEAK_TFLOPS = 78.6
 # instructions issue at 16 lanes per cycle per SIMD = FP64_PEAK_TFLOPS / 2 lane-ops/s
 FP64_ISSUE_CEILING_GITER = FP64_PEAK_TFLOPS / 2 * 1e3 / 6
 FP32_PEAK_TFLOPS = 157.3
-TUNE = (16, 24, 3)  # block_steps, handoff_lanes, tail_passes
+# block_steps, handoff_lanes, tail_passes, drain_refill: pass 1 hands its
+# stragglers to one persistent drain launch that refills lanes once 40 wait
+TUNE = (16, 24, 0, 40)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_fp64_pass1_traffic.md")
+
+
+def frame_tune(cu, tune):
+    """cu.FrameTune (wave shape 0) from (block_steps, handoff_lanes, tail_passes, drain_refill)."""
+    b, h, p, d = tune
+    return cu.FrameTune(b, h, p, 0, d)
 
 
 def log(*a):
@@ -85,7 +93,7 @@ class PipelineRun:
     def __init__(self, cu, engine, tile, precision, tune):
         self.cu = cu
         self.frame = cu.Frame(engine, [tile], precision, -1)
-        self.tune = cu.FrameTune(*tune, 0)
+        self.tune = frame_tune(cu, tune)
         self.pal = palette(cu)
         self.rgba = cu.DeviceBuffer(tile.width * tile.height * 4)
         self.pass1_ms, self.total_ms = [], []
@@ -501,12 +509,12 @@ def c1_leg(cu, engine):
     smooth values), through the frame pipeline and through the one-launch grid
     kernel.  The headline figures are the faster engine's: 97 % of
     this frame's iterations belong to the 17 % of its pixels inside the set, which
-    run all 1000 steps; the pipeline hands them on to its tail passes, while the
+    run all 1000 steps; the pipeline hands them on to its drain launch, while the
     grid kernel runs them to the end in place.  Returns the JSON entry and the
     tile with the pipeline's counts, for the CPU baseline leg's full-frame check."""
     tile = cu.make_tile(-2.0, -1.5, 1.0, 1.5, 1024, 1024, 1000, 4.0)
     fr = cu.Frame(engine, [tile], 64, -1)
-    tn = cu.FrameTune(*TUNE, 0, 0)
+    tn = frame_tune(cu, TUNE)
     for _ in range(3):
         fr.render(tn)
     ms = []
@@ -696,9 +704,9 @@ def main():
                                f"{SEED}), centre (-0.743643887037151, 0.131825904205330), width 1e-4, "
                                "maxIter 5000, bailout |z|^2>=256, smooth value + RGBA8 compose "
                                "(256-entry palette)",
-                   "kernel": "frame pipeline (pass 1 + %d tail passes + combine)" % TUNE[2],
+                   "kernel": "frame pipeline (pass 1 + the persistent drain of its straggler queue + combine)",
                    "tune": {"block_steps": TUNE[0], "handoff_lanes": TUNE[1],
-                            "tail_passes": TUNE[2]},
+                            "tail_passes": TUNE[2], "drain_refill": TUNE[3]},
                    "iterations_per_step": int(iters),
                    "render_ms_p50": round(float(np.median(render_ms)), 3),
                    "render_ms_p99": round(float(np.percentile(render_ms, 99)), 3),

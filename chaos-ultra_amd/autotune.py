@@ -35,7 +35,7 @@ PIPELINE_SPACE = {
 }
 PIPELINE_START = {"block_steps": 16, "handoff_lanes": 16, "tail_passes": 5, "wave_shape": 0,
                   "drain_refill": 0}
-DRAIN_ONLY_REFILL = (16, 32)  # tried after the descent with tail_passes 0
+DRAIN_ONLY_REFILL = (24, 32, 40, 48)  # tried after the descent with tail_passes 0
 
 PERSISTENT_SPACE = {
     "block_steps": (16, 8, 32),

@@ -194,8 +194,14 @@ void enqueue_render(cu_frame f, const cu_frame_tune &tn, const uint32_t *d_palet
         uint32_t *grab = counters + (kMaxTailPasses + 1) * kSubQueues;
         uint32_t refill = tn.drain_refill;
         void *params[] = {&ap, &q, &fills, &grab, &refill};
-        // 256 threads per block: the kernel's queue prefix assumes it
-        launch(f->engine, drain, dim3(cus * 5), dim3(256), params, stream, false);
+        // 256 threads per block: the kernel's queue prefix assumes it.  One
+        // resident set of blocks: as many per CU as its registers allow, so every
+        // wave slot holds a refilling wave and none waits behind the grid.
+        int per_cu = 0;
+        check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drain->function, 256, 0),
+              "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+        launch(f->engine, drain, dim3(cus * (uint32_t)std::max(1, per_cu)), dim3(256), params,
+               stream, false);
     }
     {
         uint64_t total = f->total_pixels;

@@ -61,6 +61,11 @@ static_assert(sizeof(PipeArgs) == 120, "PipeArgs layout");
 // sub-queue, so appends from a million waves spread over 1024 addresses instead
 // of serialising on one counter.
 constexpr uint kSubQueues = 1024;
+// Queue chunks a drain wave takes per atomic on the shared chunk counter.  One
+// address takes every grab: at one chunk per grab, the grabs of a whole pass-1
+// queue (80 K chunks for the 4096^2 4-spp frame) serialised there and doubled
+// the drain's time; at 4 the last grabs still share out evenly.
+constexpr uint kDrainGrab = 4;
 
 // Result of a pending lane (its block failed, or it is within K steps of
 // max_iter, or it cannot use blocks): the exact stop, the count in the
@@ -363,9 +368,9 @@ __device__ void passq(const PipeArgs a, const Straggler<Real> *in_queue, const u
 // Drain: one persistent launch runs every orbit left in the pass-1 queue to its
 // end, refilling idle lanes from the queue (the lane-refill scheme of
 // escape_tiles_persistent applied to queued orbits), so the longest orbits
-// start at once instead of after several hand-offs.  A wave takes one dense
-// queue chunk (build_queue_index) per atomic on `grab` and hands its records to
-// idle lanes.  Once `refill` lanes wait, the
+// start at once instead of after several hand-offs.  A wave takes kDrainGrab
+// dense queue chunks (build_queue_index) per atomic on `grab` and hands their
+// records to idle lanes.  Once `refill` lanes wait, the
 // pending lanes finish (exact stop, smooth value, per-sample store) and idle
 // lanes take new records.  Every wave leaves when the chunks are exhausted and
 // its lanes are idle; there is no barrier after the prefix.
@@ -380,12 +385,14 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
     int mode = 0;  // 0 idle, 1 running unchecked blocks, 2 pending
     bool replay = false;
     Real x = 0, y = 0, cx = 0, cy = 0, sx = 0, sy = 0, mx = 0, my = 0, bsq = 4;
+    Real x2 = 0, y2 = 0;  // squares of the current state, carried from step to step
     uint i = 0, i0 = 0, tile = 0, sample = 0, max_iter = 0;
     double smooth_c = 1;
     bool julia = false, blocks = false;
     uint64_t executed = 0;
 
     uint cq = 0, cur = 0, end = 0;  // wave-uniform record cursor in sub-queue cq
+    uint next = 0, last = 0;        // wave-uniform: chunks [next, last) taken by this wave
     bool exhausted = false;
 
     for (;;) {
@@ -393,13 +400,17 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
         uint64_t idle = cu::ballot(mode == 0);
         while (idle != 0 && !exhausted) {
             if (cur >= end) {
-                uint c = 0;
-                if (lane == 0) c = atomicAdd(grab, 1u);
-                c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
-                if (c >= total_chunks) {
-                    exhausted = true;
-                    break;
+                if (next >= last) {
+                    uint c0 = 0;
+                    if (lane == 0) c0 = atomicAdd(grab, kDrainGrab);
+                    next = __builtin_amdgcn_readfirstlane(__shfl(c0, 0, 64));
+                    last = min(next + kDrainGrab, total_chunks);
+                    if (next >= total_chunks) {
+                        exhausted = true;
+                        break;
+                    }
                 }
+                const uint c = next++;
                 cq = chunk_queue(ix, c);
                 cur = (c - ix.prefix[cq]) * 64u;
                 end = min(cur + 64u, ix.fill[cq]);
@@ -414,6 +425,8 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
                 i0 = i;
                 replay = false;
                 mode = (blocks && max_iter - i >= (uint)K) ? 1 : 2;
+                x2 = x * x;
+                y2 = y * y;
             }
             cur += take;
             idle = cu::ballot(mode == 0);
@@ -426,12 +439,12 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
                 sx = x;
                 sy = y;
 #pragma unroll
-                for (int k = 0; k < K / 2; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
+                for (int k = 0; k < K / 2; ++k) cu::quad_step_sq<Real, kFused>(x, y, x2, y2, cx, cy);
                 mx = x;
                 my = y;
 #pragma unroll
-                for (int k = K / 2; k < K; ++k) cu::quad_step<Real, kFused>(x, y, cx, cy);
-                if (x * x + y * y < bsq) {
+                for (int k = K / 2; k < K; ++k) cu::quad_step_sq<Real, kFused>(x, y, x2, y2, cx, cy);
+                if (x2 + y2 < bsq) {
                     i += K;
                     if (max_iter - i < (uint)K) mode = 2;
                 } else {

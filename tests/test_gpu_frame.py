@@ -36,7 +36,7 @@ def render(cu, engine, tiles, precision, variant, tune, palette=None):
 # what the tail passes (all handing off) leave in one persistent launch
 @pytest.mark.parametrize("tune", [(16, 16, 3, 0), (8, 16, 3, 0), (16, 1, 1, 0), (16, 32, 8, 0),
                                   (8, 4, 2, 0), (16, 24, 0, 16), (8, 32, 0, 64), (16, 8, 0, 1),
-                                  (16, 24, 2, 16), (8, 8, 7, 32)])
+                                  (16, 24, 2, 16), (8, 8, 7, 32), (16, 24, 0, 40), (16, 16, 0, 48)])
 @pytest.mark.parametrize("precision", [64, 32])
 def test_c2_frame_pipeline_vs_oracle(gpu, engine, oracle, tune, precision):
     cu = gpu
@@ -62,7 +62,7 @@ def test_c2_full_frame_bands_vs_oracle(gpu, engine, oracle, precision):
                         seed=20261202)
     pal = ((np.arange(256, dtype=np.uint32) * 0x00010305) | 0xFF000000).astype(np.uint32)
     counts, smooth, it, rgba = render(cu, engine, [tile], precision, -1,
-                                      cu.FrameTune(16, 24, 3, 0), palette=pal)
+                                      cu.FrameTune(16, 24, 0, 0, 40), palette=pal)
     counts, smooth, rgba = (a.reshape(4096, 4096) for a in (counts, smooth, rgba))
     for r0 in (0, 1856, 3712):
         want_c, want_s, _ = oracle.escape_tile(tile, precision, rows=(r0, r0 + 384))
