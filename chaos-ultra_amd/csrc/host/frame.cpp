@@ -81,6 +81,8 @@ struct cu_frame_s {
     bool replayed = false;                 // last render was a graph replay (no pass-1 event)
     hipGraph_t graph = nullptr;            // captured render (cu_frame_capture)
     hipGraphExec_t graph_exec = nullptr;
+    cu_function drain_fn = nullptr;        // drain kernel whose occupancy is cached below
+    int drain_blocks_per_cu = 0;           // resident 256-thread drain blocks per CU
 };
 
 namespace cu {
@@ -106,7 +108,7 @@ void fill(cu_frame f, cu_stream s, void *p, uint32_t value, uint64_t words) {
 
 cu_frame_tune checked_tune(const cu_frame_tune *tune, const uint32_t *d_palette,
                            uint32_t palette_length, const uint32_t *d_rgba) {
-    cu_frame_tune tn{16, 24, 3, 0, 0};
+    cu_frame_tune tn{16, 24, 0, 0, 40};  // pass 1, then one drain of its whole queue
     if (tune) tn = *tune;
     if (tn.block_steps != 8 && tn.block_steps != 16)
         fail(CU_ERR_ILLEGAL_ARGUMENT, "block_steps must be 8 or 16");
@@ -197,10 +199,14 @@ void enqueue_render(cu_frame f, const cu_frame_tune &tn, const uint32_t *d_palet
         // 256 threads per block: the kernel's queue prefix assumes it.  One
         // resident set of blocks: as many per CU as its registers allow, so every
         // wave slot holds a refilling wave and none waits behind the grid.
-        int per_cu = 0;
-        check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drain->function, 256, 0),
-              "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
-        launch(f->engine, drain, dim3(cus * (uint32_t)std::max(1, per_cu)), dim3(256), params,
+        if (f->drain_fn != drain) {
+            int per_cu = 0;
+            check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drain->function, 256, 0),
+                  "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+            f->drain_fn = drain;
+            f->drain_blocks_per_cu = std::max(1, per_cu);
+        }
+        launch(f->engine, drain, dim3(cus * (uint32_t)f->drain_blocks_per_cu), dim3(256), params,
                stream, false);
     }
     {

@@ -372,7 +372,8 @@ typedef struct cu_frame_s *cu_frame;
 int cu_frame_create(cu_engine e, const cu_tile_desc *tiles, uint32_t n_tiles, int precision,
                     int variant, cu_frame *out);
 int cu_frame_destroy(cu_frame f);
-/* d_palette / d_rgba may be NULL (no compose).  tune may be NULL (defaults). */
+/* d_palette / d_rgba may be NULL (no compose).  tune may be NULL: block_steps 16, handoff_lanes 24,
+   tail_passes 0, wave_shape 0, drain_refill 40 (pass 1, then one drain of its whole queue). */
 int cu_frame_render(cu_frame f, const cu_frame_tune *tune, const uint32_t *d_palette,
                     uint32_t palette_length, uint32_t *d_rgba, cu_stream stream);
 /* Device pointers of the pixel planes (total_pixels elements each). */
