@@ -26,7 +26,7 @@ sample grid kernel and the FP32 pipeline on the same frame, and the oracle's CPU
 rate on a bounded row band of the same frame on the host's cores.
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-variants]
-                [--leg uneven|deep|serving|renderer]
+                [--leg c1|uneven|deep|serving|serving_capacity|renderer]
 """
 from __future__ import annotations
 
@@ -573,7 +573,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the grid / FP32 variants and the uneven-tile and deep-zoom legs")
-    ap.add_argument("--leg", choices=["all", "uneven", "deep", "serving", "serving_capacity", "renderer"],
+    ap.add_argument("--leg", choices=["all", "c1", "uneven", "deep", "serving", "serving_capacity",
+                                      "renderer"],
                     default="all",
                     help="run one leg only and print its JSON (used for profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -589,7 +590,8 @@ def main():
     cu.init(0)
     engine = cu.Engine()
     if args.leg != "all":
-        leg = {"uneven": lambda: uneven_leg(cu, engine), "deep": lambda: deep_leg(cu, engine),
+        leg = {"c1": lambda: c1_leg(cu, engine)[0],
+               "uneven": lambda: uneven_leg(cu, engine), "deep": lambda: deep_leg(cu, engine),
                "serving": lambda: serving_leg(cu), "renderer": lambda: renderer_leg(cu),
                "serving_capacity": lambda: serving_capacity_leg(cu)}[args.leg]()
         calibrate_copy(cu, args.calibrate_copy_mib)

@@ -182,7 +182,14 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
     uint max_iter = 0;
     uint64_t executed = 0;
 
-    uint64_t cur = 0, end = 0;  // wave-uniform chunk cursor
+    // Wave-uniform chunk cursor.  A wave's first chunk follows from its number,
+    // so the launch does not start with every wave's atomic on the one counter;
+    // the counter hands out the chunks after all of the first ones.
+    const uint64_t wave_global = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                                                (threadIdx.x >> 6));
+    const uint64_t first_items = (uint64_t)gridDim.x * (blockDim.x >> 6) * chunk;
+    uint64_t cur = wave_global * chunk;
+    uint64_t end = min(cur + chunk, total);
     bool exhausted = false;
 
     for (;;) {
@@ -191,8 +198,8 @@ __device__ void escape_tiles_persistent(const cu_tile_desc *__restrict__ tiles, 
         while (idle != 0 && !exhausted) {
             if (cur >= end) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(queue, (unsigned long long)chunk);
-                base = __shfl(base, 0, 64);
+                if (lane == 0 && first_items < total) base = atomicAdd(queue, (unsigned long long)chunk);
+                base = first_items + __shfl(base, 0, 64);
                 if (base >= total) {
                     exhausted = true;
                     break;

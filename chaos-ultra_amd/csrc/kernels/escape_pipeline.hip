@@ -392,7 +392,15 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
     uint64_t executed = 0;
 
     uint cq = 0, cur = 0, end = 0;  // wave-uniform record cursor in sub-queue cq
-    uint next = 0, last = 0;        // wave-uniform: chunks [next, last) taken by this wave
+    // Wave-uniform: chunks [next, last) taken by this wave.  A wave's first
+    // kDrainGrab chunks follow from its number, so the launch does not start
+    // with every wave's atomic on the one counter; later grabs come after all
+    // of the first ones.
+    const uint wave_global = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                                            (threadIdx.x >> 6));
+    const uint first_chunks = gridDim.x * (blockDim.x >> 6) * kDrainGrab;
+    uint next = wave_global * kDrainGrab;
+    uint last = min(next + kDrainGrab, total_chunks);
     bool exhausted = false;
 
     for (;;) {
@@ -402,13 +410,13 @@ __device__ void drain(const PipeArgs a, const Straggler<Real> *queue, const uint
             if (cur >= end) {
                 if (next >= last) {
                     uint c0 = 0;
-                    if (lane == 0) c0 = atomicAdd(grab, kDrainGrab);
-                    next = __builtin_amdgcn_readfirstlane(__shfl(c0, 0, 64));
+                    if (lane == 0 && first_chunks < total_chunks) c0 = atomicAdd(grab, kDrainGrab);
+                    next = first_chunks + __builtin_amdgcn_readfirstlane(__shfl(c0, 0, 64));
                     last = min(next + kDrainGrab, total_chunks);
-                    if (next >= total_chunks) {
-                        exhausted = true;
-                        break;
-                    }
+                }
+                if (next >= total_chunks) {
+                    exhausted = true;
+                    break;
                 }
                 const uint c = next++;
                 cq = chunk_queue(ix, c);
