@@ -53,12 +53,15 @@ constexpr uint kZoomingFlag = 1u << 4;
 constexpr uint kZoomingInFlag = 1u << 5;
 
 // getImageIndexes (fractalRendererGeneric.cu:36-53): thread -> pixel so that
-// each 32-thread group covers an 8x4 rectangle of the 32x32 block tile.
+// each 32-thread group covers an 8x4 rectangle of the 32x32 block tile.  The
+// tile of a block is its row of the grid rotated (cu::skewed_block_x), which
+// spreads the blocks a CU receives over the frame; a tile and its vote groups
+// are the reference's, only the block that computes them differs.
 __device__ __forceinline__ Point<uint> image_indexes() {
     const uint t = threadIdx.x + threadIdx.y * 32u;
     const uint warp = t >> 5, lane = t & 31u;
     Point<uint> p;
-    p.x = blockIdx.x * 32u + (warp & 3u) * 8u + (lane >> 4) * 4u + (lane & 3u);
+    p.x = cu::skewed_block_x() * 32u + (warp & 3u) * 8u + (lane >> 4) * 4u + (lane & 3u);
     p.y = blockIdx.y * 32u + (warp >> 2) * 4u + ((lane >> 2) & 3u);
     return p;
 }
@@ -133,7 +136,7 @@ __device__ __forceinline__ float dispersion(const float (&samples)[kAdaptiveThre
 __device__ __forceinline__ Point<uint> image_indexes_at(uint t) {
     const uint warp = t >> 5, lane = t & 31u;
     Point<uint> p;
-    p.x = blockIdx.x * 32u + (warp & 3u) * 8u + (lane >> 4) * 4u + (lane & 3u);
+    p.x = cu::skewed_block_x() * 32u + (warp & 3u) * 8u + (lane >> 4) * 4u + (lane & 3u);
     p.y = blockIdx.y * 32u + (warp >> 2) * 4u + ((lane >> 2) & 3u);
     return p;
 }
