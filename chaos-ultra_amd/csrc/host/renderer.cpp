@@ -113,6 +113,13 @@ const char *const kKernelNames[] = {"fractalRenderMainFloat",     "fractalRender
                                     "fractalRenderAdvancedFloat", "fractalRenderAdvancedDouble",
                                     "fractalRenderUnderSampled",  "compose",
                                     "debug"};
+// The same four render kernels for 32 x 8-thread blocks (this build's modules
+// export them; a module without them renders through the kernels above).
+const char *const kStripKernelNames[] = {"fractalRenderMainFloatStrip",
+                                         "fractalRenderMainDoubleStrip",
+                                         "fractalRenderAdvancedFloatStrip",
+                                         "fractalRenderAdvancedDoubleStrip"};
+constexpr int kStripRows = 8;  // rows of a strip block
 
 class Module {
   public:
@@ -140,6 +147,10 @@ class Module {
             }
             kernels_[k] = f;
         }
+        for (const char *k : kStripKernelNames) {
+            cu_function f = nullptr;
+            if (cu_module_get_function(module_, k, &f) == CU_OK) kernels_[k] = f;
+        }
         initialized_ = true;
     }
 
@@ -152,6 +163,11 @@ class Module {
         auto it = kernels_.find(k);
         if (it == kernels_.end()) fail(CU_ERR_ILLEGAL_ARGUMENT, "No such kernel available: " + k);
         return it->second;
+    }
+    // An optional kernel of the module, or null.
+    cu_function kernel_if(const std::string &k) const {
+        auto it = kernels_.find(k);
+        return it == kernels_.end() ? nullptr : it->second;
     }
 
     void close() {  // (:140-147)
@@ -325,6 +341,7 @@ struct RectT {
 
 struct RenderKernel {
     cu_function fn = nullptr;
+    cu_function strip = nullptr;  // the same kernel for 32 x 8-thread blocks, if the module has it
     bool is_float = false;
     bool advanced = false;
     int width = 0, height = 0;
@@ -377,7 +394,9 @@ struct RenderKernel {
         }
     }
 
-    // Launch with the reference's geometry and synchronise.
+    // Launch and synchronise: the strip kernel over ceil(W/32) x ceil(H/8) blocks
+    // of 32 x 8 threads when the module has it (the same pixels, vote groups and
+    // results), else the reference's geometry.
     void launch() {
         PointT<uint32_t> size{(uint32_t)width, (uint32_t)height};
         RectT<double> rd{seg.lbx, seg.lby, seg.rtx, seg.rty};
@@ -391,7 +410,13 @@ struct RenderKernel {
                           reused, &in, &in_pitch, &focus};
         unsigned gx = (unsigned)((width + kBlockDim - 1) / kBlockDim);
         unsigned gy = (unsigned)((height + kBlockDim - 1) / kBlockDim);
-        int rc = cu_launch_kernel(fn, gx, gy, 1, kBlockDim, kBlockDim, 1, 0, params);
+        const int64_t strip_rows = ((int64_t)height + kStripRows - 1) / kStripRows;
+        int rc;
+        if (strip && strip_rows <= kMaxGridDim)
+            rc = cu_launch_kernel(strip, gx, (unsigned)strip_rows, 1, kBlockDim, kStripRows, 1, 0,
+                                  params);
+        else
+            rc = cu_launch_kernel(fn, gx, gy, 1, kBlockDim, kBlockDim, 1, 0, params);
         if (rc == CU_OK) rc = cu_ctx_synchronize();
         if (rc != CU_OK) fail(rc, cu_last_error());
     }
@@ -422,12 +447,16 @@ class Renderer {
   public:
     explicit Renderer(Module *module) : module_(module) {
         main_f_.fn = module->kernel("fractalRenderMainFloat");
+        main_f_.strip = module->kernel_if("fractalRenderMainFloatStrip");
         main_f_.is_float = true;
         main_d_.fn = module->kernel("fractalRenderMainDouble");
+        main_d_.strip = module->kernel_if("fractalRenderMainDoubleStrip");
         adv_f_.fn = module->kernel("fractalRenderAdvancedFloat");
+        adv_f_.strip = module->kernel_if("fractalRenderAdvancedFloatStrip");
         adv_f_.is_float = true;
         adv_f_.advanced = true;
         adv_d_.fn = module->kernel("fractalRenderAdvancedDouble");
+        adv_d_.strip = module->kernel_if("fractalRenderAdvancedDoubleStrip");
         adv_d_.advanced = true;
         compose_ = module->kernel("compose");
         debug_ = module->kernel("debug");

@@ -47,6 +47,7 @@ namespace {
 
 constexpr uint32_t kMaxSlots = 16;
 constexpr int kBlockDim = 32;  // reference launch geometry (CudaFractalRenderer.java:36-37)
+constexpr int kStripRows = 8;  // rows of a block of the render kernels' 32 x 8-thread form
 
 template <class T>
 struct Pt {
@@ -99,6 +100,8 @@ struct cu_serve_s {
     cu_module module = nullptr;
     cu_function main_f = nullptr, main_d = nullptr, adv_f = nullptr, adv_d = nullptr,
                 compose = nullptr, checksum = nullptr;
+    // the render kernels for 32 x 8-thread blocks (null if the module lacks them)
+    cu_function main_fs = nullptr, main_ds = nullptr, adv_fs = nullptr, adv_ds = nullptr;
     void *d_module_iters = nullptr;  // module symbol cu_iteration_slots
     cu_serve_config cfg{};
     std::vector<Slot> slots;
@@ -209,6 +212,21 @@ void launch_module(cu_function f, unsigned gx, unsigned gy, void **params, hipSt
           (std::string("hipModuleLaunchKernel ") + f->name).c_str());
 }
 
+// A render kernel over a W x H frame: its 32 x 8-thread form over ceil(W/32) x
+// ceil(H/8) blocks when the module has one (the same pixels and results), else
+// the reference's 32 x 32 blocks.
+void launch_render(cu_function f, cu_function strip, uint32_t W, uint32_t H, void **params,
+                   hipStream_t st) {
+    const unsigned gx = (W + kBlockDim - 1) / kBlockDim;
+    if (strip) {
+        check(hipModuleLaunchKernel(strip->function, gx, (H + kStripRows - 1) / kStripRows, 1,
+                                    kBlockDim, kStripRows, 1, 0, st, params, nullptr),
+              (std::string("hipModuleLaunchKernel ") + strip->name).c_str());
+    } else {
+        launch_module(f, gx, (H + kBlockDim - 1) / kBlockDim, params, st);
+    }
+}
+
 // One frame of session ss on slot sl (asynchronous on the slot's stream).
 void dispatch(cu_serve_s *s, Session &ss, Slot &sl, const Request &rq) {
     const cu_frame_request &r = rq.req;
@@ -246,7 +264,8 @@ void dispatch(cu_serve_s *s, Session &ss, Slot &sl, const Request &rq) {
             uint32_t flags = CU_FLAG_ADAPTIVE_SS;
             void *params[] = {&out, &pitch, &size, fp32 ? (void *)&rf : (void *)&rd, &maxit,
                               &max_ss, &flags};
-            launch_module(fp32 ? s->main_f : s->main_d, gx, gy, params, st);
+            launch_render(fp32 ? s->main_f : s->main_d, fp32 ? s->main_fs : s->main_ds, W, H, params,
+                          st);
             path = 1;
         } else {  // renderFast: advanced kernel primary -> secondary, then swap
             void *in = ss.buf[ss.primary];
@@ -262,7 +281,7 @@ void dispatch(cu_serve_s *s, Session &ss, Slot &sl, const Request &rq) {
             void *params[] = {&out, &out_pitch, &size, fp32 ? (void *)&rf : (void *)&rd, &maxit,
                               &max_ss, &flags, fp32 ? (void *)&of : (void *)&od, &in, &in_pitch,
                               &focus};
-            launch_module(fp32 ? s->adv_f : s->adv_d, gx, gy, params, st);
+            launch_render(fp32 ? s->adv_f : s->adv_d, fp32 ? s->adv_fs : s->adv_ds, W, H, params, st);
             ss.primary = 1 - ss.primary;
             path = 0;
         }
@@ -394,6 +413,10 @@ cu_function module_fn(cu_module m, const char *name) {
     if (cu_module_get_function(m, name, &f) != CU_OK) fail(CU_ERR_INITIALIZATION, cu_last_error());
     return f;
 }
+cu_function module_fn_if(cu_module m, const char *name) {
+    cu_function f = nullptr;
+    return cu_module_get_function(m, name, &f) == CU_OK ? f : nullptr;
+}
 
 }  // namespace
 }  // namespace cu
@@ -427,6 +450,10 @@ int cu_serve_create(const char *kernels_dir, const cu_serve_config *cfg, cu_serv
         s->main_d = module_fn(s->module, "fractalRenderMainDouble");
         s->adv_f = module_fn(s->module, "fractalRenderAdvancedFloat");
         s->adv_d = module_fn(s->module, "fractalRenderAdvancedDouble");
+        s->main_fs = module_fn_if(s->module, "fractalRenderMainFloatStrip");
+        s->main_ds = module_fn_if(s->module, "fractalRenderMainDoubleStrip");
+        s->adv_fs = module_fn_if(s->module, "fractalRenderAdvancedFloatStrip");
+        s->adv_ds = module_fn_if(s->module, "fractalRenderAdvancedDoubleStrip");
         s->compose = module_fn(s->module, "compose");
         s->checksum = function(s->engine, "checksumU32");
         size_t bytes = 0;

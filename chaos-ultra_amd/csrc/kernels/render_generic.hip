@@ -56,13 +56,15 @@ constexpr uint kZoomingInFlag = 1u << 5;
 // each 32-thread group covers an 8x4 rectangle of the 32x32 block tile.  The
 // tile of a block is its row of the grid rotated (cu::skewed_block_x), which
 // spreads the blocks a CU receives over the frame; a tile and its vote groups
-// are the reference's, only the block that computes them differs.
+// are the reference's, only the block that computes them differs.  A block of
+// 32 x 8 threads (the *Strip kernels) covers 32 x 8 pixels the same way: the
+// same 8x4 rectangles, and the same two of them in each wave.
 __device__ __forceinline__ Point<uint> image_indexes() {
     const uint t = threadIdx.x + threadIdx.y * 32u;
     const uint warp = t >> 5, lane = t & 31u;
     Point<uint> p;
     p.x = cu::skewed_block_x() * 32u + (warp & 3u) * 8u + (lane >> 4) * 4u + (lane & 3u);
-    p.y = blockIdx.y * 32u + (warp >> 2) * 4u + ((lane >> 2) & 3u);
+    p.y = blockIdx.y * blockDim.y + (warp >> 2) * 4u + ((lane >> 2) & 3u);
     return p;
 }
 
@@ -137,7 +139,7 @@ __device__ __forceinline__ Point<uint> image_indexes_at(uint t) {
     const uint warp = t >> 5, lane = t & 31u;
     Point<uint> p;
     p.x = cu::skewed_block_x() * 32u + (warp & 3u) * 8u + (lane >> 4) * 4u + (lane & 3u);
-    p.y = blockIdx.y * 32u + (warp >> 2) * 4u + ((lane >> 2) & 3u);
+    p.y = blockIdx.y * blockDim.y + (warp >> 2) * 4u + ((lane >> 2) & 3u);
     return p;
 }
 
@@ -429,14 +431,14 @@ __device__ __forceinline__ bool fused_valid(Rectangle<Real> image, Point<uint> g
     }
 }
 
-template <class Real>
+template <class Real, uint kWaves>
 __device__ void render_main(pixel_info_t *out, long out_pitch, Point<uint> size,
                             Rectangle<Real> image, uint max_iter, float max_ss, uint flags) {
     const Point<uint> idx = image_indexes();
     const bool inside = idx.x < size.x && idx.y < size.y;
     float sample_count = max_ss;
     uint64_t executed = 0;
-    __shared__ WaveLds lds_waves[16];
+    __shared__ WaveLds lds_waves[kWaves];
     WaveLds &lds = lds_waves[(threadIdx.x + threadIdx.y * blockDim.x) >> 6];
     const bool adaptive = (flags & kAdaptiveFlag) != 0;
     const uint r = fused_valid<Real>(image, size)
@@ -519,7 +521,7 @@ __device__ __forceinline__ void foveation(Point<uint> pixel, Point<uint> focus, 
     }
 }
 
-template <class Real>
+template <class Real, uint kWaves>
 __device__ void render_advanced(pixel_info_t *out, long out_pitch, Point<uint> size,
                                 Rectangle<Real> image, uint max_iter, float max_ss, uint flags,
                                 Rectangle<Real> image_reused, const pixel_info_t *in,
@@ -570,7 +572,7 @@ __device__ void render_advanced(pixel_info_t *out, long out_pitch, Point<uint> s
         }
     }
     uint64_t executed = 0;
-    __shared__ WaveLds lds_waves[16];
+    __shared__ WaveLds lds_waves[kWaves];
     WaveLds &lds = lds_waves[(threadIdx.x + threadIdx.y * blockDim.x) >> 6];
     const bool adaptive = (flags & kAdaptiveFlag) != 0;
     const uint r = fused_valid<Real>(image, size)
@@ -605,35 +607,48 @@ __device__ __forceinline__ uint colorize_sample_count(uint sample_count, uint fu
 
 extern "C" __global__ void init() {}
 
-extern "C" __global__ __launch_bounds__(1024) void fractalRenderMainFloat(
-    pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<float> image,
-    uint maxIterations, float maxSuperSampling, uint flags) {
-    render_main<float>(output, outputPitch, outputSize, image, maxIterations, maxSuperSampling,
-                       flags);
-}
+#define CU_RENDER_KERNELS(SUFFIX, BOUNDS, BOUNDS_ADV_DOUBLE, WAVES)                               \
+    extern "C" __global__ BOUNDS void fractalRenderMainFloat##SUFFIX(                               \
+        pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<float> image,     \
+        uint maxIterations, float maxSuperSampling, uint flags) {                                   \
+        render_main<float, WAVES>(output, outputPitch, outputSize, image, maxIterations,            \
+                                  maxSuperSampling, flags);                                         \
+    }                                                                                               \
+    extern "C" __global__ BOUNDS void fractalRenderMainDouble##SUFFIX(                              \
+        pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<double> image,    \
+        uint maxIterations, float maxSuperSampling, uint flags) {                                   \
+        render_main<double, WAVES>(output, outputPitch, outputSize, image, maxIterations,           \
+                                   maxSuperSampling, flags);                                        \
+    }                                                                                               \
+    extern "C" __global__ BOUNDS void fractalRenderAdvancedFloat##SUFFIX(                           \
+        pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<float> image,     \
+        uint maxIterations, float maxSuperSampling, uint flags, Rectangle<float> imageReused,       \
+        const pixel_info_t *input, long inputPitch, Point<uint> focus) {                            \
+        render_advanced<float, WAVES>(output, outputPitch, outputSize, image, maxIterations,        \
+                                      maxSuperSampling, flags, imageReused, input, inputPitch,      \
+                                      focus);                                                       \
+    }                                                                                               \
+    extern "C" __global__ BOUNDS_ADV_DOUBLE void fractalRenderAdvancedDouble##SUFFIX(               \
+        pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<double> image,    \
+        uint maxIterations, float maxSuperSampling, uint flags, Rectangle<double> imageReused,      \
+        const pixel_info_t *input, long inputPitch, Point<uint> focus) {                            \
+        render_advanced<double, WAVES>(output, outputPitch, outputSize, image, maxIterations,       \
+                                       maxSuperSampling, flags, imageReused, input, inputPitch,     \
+                                       focus);                                                      \
+    }
 
-extern "C" __global__ __launch_bounds__(1024) void fractalRenderMainDouble(
-    pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<double> image,
-    uint maxIterations, float maxSuperSampling, uint flags) {
-    render_main<double>(output, outputPitch, outputSize, image, maxIterations, maxSuperSampling,
-                        flags);
-}
+// The reference's kernels: 32 x 32-thread blocks over ceil(W/32) x ceil(H/32)
+// (CudaFractalRenderer.java:242-258), 16 waves and their LDS per block.
+CU_RENDER_KERNELS(, __launch_bounds__(1024), __launch_bounds__(1024), 16u)
 
-extern "C" __global__ __launch_bounds__(1024) void fractalRenderAdvancedFloat(
-    pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<float> image,
-    uint maxIterations, float maxSuperSampling, uint flags, Rectangle<float> imageReused,
-    const pixel_info_t *input, long inputPitch, Point<uint> focus) {
-    render_advanced<float>(output, outputPitch, outputSize, image, maxIterations,
-                           maxSuperSampling, flags, imageReused, input, inputPitch, focus);
-}
-
-extern "C" __global__ __launch_bounds__(1024) void fractalRenderAdvancedDouble(
-    pixel_info_t *output, long outputPitch, Point<uint> outputSize, Rectangle<double> image,
-    uint maxIterations, float maxSuperSampling, uint flags, Rectangle<double> imageReused,
-    const pixel_info_t *input, long inputPitch, Point<uint> focus) {
-    render_advanced<double>(output, outputPitch, outputSize, image, maxIterations,
-                            maxSuperSampling, flags, imageReused, input, inputPitch, focus);
-}
+// The same kernels for 32 x 8-thread blocks over ceil(W/32) x ceil(H/8), which
+// the renderer layer launches: the same pixels, vote groups and results.  A
+// 1024-thread block's LDS and wave slots free only when its last wave ends, and
+// with 70 VGPRs a CU holds 28 waves, so the next such block waits for 4 of them:
+// about 3 waves per SIMD stay resident.  Blocks of 4 waves keep every wave slot
+// filled (7 blocks per CU, 133 KB of LDS).  The FP64 advanced kernel needs 78
+// VGPRs: it is held to 6 waves per SIMD rather than spill.
+CU_RENDER_KERNELS(Strip, __launch_bounds__(256, 7), __launch_bounds__(256, 6), 4u)
 
 // compose (fractalRendererGeneric.cu:455-475).  The palette is staged in LDS
 // (up to kPaletteLds entries; longer palettes are read from global memory) and
