@@ -113,13 +113,13 @@ const char *const kKernelNames[] = {"fractalRenderMainFloat",     "fractalRender
                                     "fractalRenderAdvancedFloat", "fractalRenderAdvancedDouble",
                                     "fractalRenderUnderSampled",  "compose",
                                     "debug"};
-// The same four render kernels for 32 x 8-thread blocks (this build's modules
+// The same four render kernels for 32 x 4-thread blocks (this build's modules
 // export them; a module without them renders through the kernels above).
 const char *const kStripKernelNames[] = {"fractalRenderMainFloatStrip",
                                          "fractalRenderMainDoubleStrip",
                                          "fractalRenderAdvancedFloatStrip",
                                          "fractalRenderAdvancedDoubleStrip"};
-constexpr int kStripRows = 8;  // rows of a strip block
+constexpr int kStripRows = 4;  // rows of a strip block
 
 class Module {
   public:
@@ -341,7 +341,7 @@ struct RectT {
 
 struct RenderKernel {
     cu_function fn = nullptr;
-    cu_function strip = nullptr;  // the same kernel for 32 x 8-thread blocks, if the module has it
+    cu_function strip = nullptr;  // the same kernel for 32 x 4-thread blocks, if the module has it
     bool is_float = false;
     bool advanced = false;
     int width = 0, height = 0;
@@ -394,8 +394,8 @@ struct RenderKernel {
         }
     }
 
-    // Launch and synchronise: the strip kernel over ceil(W/32) x ceil(H/8) blocks
-    // of 32 x 8 threads when the module has it (the same pixels, vote groups and
+    // Launch and synchronise: the strip kernel over ceil(W/32) x ceil(H/4) blocks
+    // of 32 x 4 threads when the module has it (the same pixels, vote groups and
     // results), else the reference's geometry.
     void launch() {
         PointT<uint32_t> size{(uint32_t)width, (uint32_t)height};

@@ -47,7 +47,7 @@ namespace {
 
 constexpr uint32_t kMaxSlots = 16;
 constexpr int kBlockDim = 32;  // reference launch geometry (CudaFractalRenderer.java:36-37)
-constexpr int kStripRows = 8;  // rows of a block of the render kernels' 32 x 8-thread form
+constexpr int kStripRows = 4;  // rows of a block of the render kernels' 32 x 4-thread form
 
 template <class T>
 struct Pt {
@@ -100,7 +100,7 @@ struct cu_serve_s {
     cu_module module = nullptr;
     cu_function main_f = nullptr, main_d = nullptr, adv_f = nullptr, adv_d = nullptr,
                 compose = nullptr, checksum = nullptr;
-    // the render kernels for 32 x 8-thread blocks (null if the module lacks them)
+    // the render kernels for 32 x 4-thread blocks (null if the module lacks them)
     cu_function main_fs = nullptr, main_ds = nullptr, adv_fs = nullptr, adv_ds = nullptr;
     void *d_module_iters = nullptr;  // module symbol cu_iteration_slots
     cu_serve_config cfg{};
@@ -212,8 +212,8 @@ void launch_module(cu_function f, unsigned gx, unsigned gy, void **params, hipSt
           (std::string("hipModuleLaunchKernel ") + f->name).c_str());
 }
 
-// A render kernel over a W x H frame: its 32 x 8-thread form over ceil(W/32) x
-// ceil(H/8) blocks when the module has one (the same pixels and results), else
+// A render kernel over a W x H frame: its 32 x 4-thread form over ceil(W/32) x
+// ceil(H/4) blocks when the module has one (the same pixels and results), else
 // the reference's 32 x 32 blocks.
 void launch_render(cu_function f, cu_function strip, uint32_t W, uint32_t H, void **params,
                    hipStream_t st) {

@@ -57,7 +57,7 @@ constexpr uint kZoomingInFlag = 1u << 5;
 // tile of a block is its row of the grid rotated (cu::skewed_block_x), which
 // spreads the blocks a CU receives over the frame; a tile and its vote groups
 // are the reference's, only the block that computes them differs.  A block of
-// 32 x 8 threads (the *Strip kernels) covers 32 x 8 pixels the same way: the
+// 32 x 4 threads (the *Strip kernels) covers 32 x 4 pixels the same way: the
 // same 8x4 rectangles, and the same two of them in each wave.
 __device__ __forceinline__ Point<uint> image_indexes() {
     const uint t = threadIdx.x + threadIdx.y * 32u;
@@ -641,14 +641,14 @@ extern "C" __global__ void init() {}
 // (CudaFractalRenderer.java:242-258), 16 waves and their LDS per block.
 CU_RENDER_KERNELS(, __launch_bounds__(1024), __launch_bounds__(1024), 16u)
 
-// The same kernels for 32 x 8-thread blocks over ceil(W/32) x ceil(H/8), which
-// the renderer layer launches: the same pixels, vote groups and results.  A
-// 1024-thread block's LDS and wave slots free only when its last wave ends, and
-// with 70 VGPRs a CU holds 28 waves, so the next such block waits for 4 of them:
-// about 3 waves per SIMD stay resident.  Blocks of 4 waves keep every wave slot
-// filled (7 blocks per CU, 133 KB of LDS).  The FP64 advanced kernel needs 78
-// VGPRs: it is held to 6 waves per SIMD rather than spill.
-CU_RENDER_KERNELS(Strip, __launch_bounds__(256, 7), __launch_bounds__(256, 6), 4u)
+// The same kernels for 32 x 4-thread blocks over ceil(W/32) x ceil(H/4), which
+// the renderer layer and the serving loop launch: the same pixels, vote groups
+// and results.  A 1024-thread block's LDS and wave slots free only when its last
+// wave ends, and with 70 VGPRs a CU holds 28 waves, so the next such block waits
+// for 4 of them: about 3 waves per SIMD stay resident.  Blocks of 2 waves keep
+// the wave slots filled (14 blocks per CU, 136 KB of LDS).  The FP64 advanced
+// kernel needs 78 VGPRs: it is held to 6 waves per SIMD rather than spill.
+CU_RENDER_KERNELS(Strip, __launch_bounds__(128, 7), __launch_bounds__(128, 6), 2u)
 
 // compose (fractalRendererGeneric.cu:455-475).  The palette is staged in LDS
 // (up to kPaletteLds entries; longer palettes are read from global memory) and
