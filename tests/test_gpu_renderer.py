@@ -1,9 +1,11 @@
 """The FractalRenderer / FractalRendererProvider surface on the GPU against the oracle.
 
 renderQuality and renderFast run the module kernels (fractalRenderMain*,
-fractalRenderAdvanced*, compose) with the reference's launch geometry; their
-pixel_info_t buffers and composed RGBA8 images must equal the oracle's lock-step
-restatement bit for bit.
+fractalRenderAdvanced*, compose): the renderer layer launches their 32 x 4-thread
+form (*Strip), a host binding the driver layer the reference-named kernels with
+the reference's 32 x 32 geometry.  Their pixel_info_t buffers and composed RGBA8
+images must equal the oracle's lock-step restatement bit for bit, whichever
+geometry computed them.
 """
 import ctypes as C
 
@@ -308,6 +310,66 @@ def test_driver_layer_under_sampled_kernel(gpu, oracle):
             cu.abi.call("cu_module_write_global", mod, b"VISUALIZE_SAMPLE_COUNT",
                         C.byref(C.c_double(1.0)), C.c_size_t(8))
     finally:
+        cu.abi.check(L.cu_module_unload(mod))
+
+
+class _PointU(C.Structure):
+    _fields_ = [("x", C.c_uint), ("y", C.c_uint)]
+
+
+class _RectD(C.Structure):
+    _fields_ = [("lbx", C.c_double), ("lby", C.c_double), ("rtx", C.c_double), ("rty", C.c_double)]
+
+
+class _RectF(C.Structure):
+    _fields_ = [("lbx", C.c_float), ("lby", C.c_float), ("rtx", C.c_float), ("rty", C.c_float)]
+
+
+@pytest.mark.parametrize("prec,max_ss,flags_adaptive", [(64, 4.0, False), (64, 16.0, True),
+                                                         (32, 5.0, True)])
+def test_main_kernel_geometries_agree(gpu, oracle, prec, max_ss, flags_adaptive):
+    """fractalRenderMain* through the bare driver calls with the reference's
+    32 x 32-thread blocks over ceil(W/32) x ceil(H/32), and its strip form with
+    32 x 4-thread blocks over ceil(W/32) x ceil(H/4): the same 8x4 vote groups, so
+    the same pixel_info_t, equal to the oracle's (a height that is not a multiple
+    of 32 leaves the last reference block partly outside the image)."""
+    cu = gpu
+    L = cu.abi.load()
+    w, h = 480, 270
+    if prec == 64:
+        seg = cu.plane_segment_from_center(-0.743643887037151, 0.131825904205330, 1e-4 * h / w, w, h)
+        max_iter, rect, name = 3000, _RectD(*seg), "fractalRenderMainDouble"
+    else:
+        seg = cu.plane_segment_from_center(-0.5, 0.0, 2.0, w, h)
+        max_iter, rect, name = 1600, _RectF(*seg), "fractalRenderMainFloat"
+    flags = cu.FLAG_ADAPTIVE_SS if flags_adaptive else 0
+    mod = C.c_void_p()
+    cu.abi.check(L.cu_module_load(f"{cu.KERNELS_DIR}/mandelbrot.hsaco".encode(), C.byref(mod)))
+    dptr, pitch = C.c_void_p(), C.c_size_t()
+    cu.abi.check(L.cu_mem_alloc_pitch(C.byref(dptr), C.byref(pitch), w * 16, h, 16))
+    try:
+        got = {}
+        for kernel, rows in ((name, 32), (name + "Strip", 4)):
+            fn = C.c_void_p()
+            cu.abi.check(L.cu_module_get_function(mod, kernel.encode(), C.byref(fn)))
+            zero = np.zeros(pitch.value * h // 4, dtype=np.uint32)
+            cu.abi.check(L.cu_memcpy_htod(dptr, zero.ctypes.data_as(C.c_void_p), zero.nbytes))
+            args = [C.c_void_p(dptr.value), C.c_long(pitch.value), _PointU(w, h), rect,
+                    C.c_uint(max_iter), C.c_float(max_ss), C.c_uint(flags)]
+            params = (C.c_void_p * len(args))(*[C.cast(C.pointer(a), C.c_void_p) for a in args])
+            cu.abi.check(L.cu_launch_kernel(fn, (w + 31) // 32, (h + rows - 1) // rows, 1, 32, rows,
+                                            1, 0, params))
+            cu.abi.check(L.cu_ctx_synchronize())
+            out = np.zeros((h, w), dtype=cu.PIXEL_INFO_DTYPE)
+            cu.abi.check(L.cu_memcpy_2d_dtoh(out.ctypes.data_as(C.c_void_p), w * 16, dptr,
+                                             pitch.value, w * 16, h))
+            got[rows] = out
+        want = oracle.render_main(prec, cu.FRACTAL_MANDELBROT, (0, 0), w, h, seg, max_iter, max_ss,
+                                  flags)
+        fields_equal(got[32], want)
+        fields_equal(got[4], want)
+    finally:
+        cu.abi.check(L.cu_mem_free(dptr))
         cu.abi.check(L.cu_module_unload(mod))
 
 
