@@ -6,7 +6,7 @@ reader on every truncation and single-byte corruption of the reference's palette
 files and on headers of every colour type, depth and size; PNG output round
 trips; palettes, plane mapping, precision switch, launch geometry and tile
 set-up at edge arguments.  No device code is built or run; the test passes when
-the driver exits 0 with no sanitizer report."""
+that program exits 0 with no sanitizer report."""
 import glob
 import os
 import subprocess
